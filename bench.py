@@ -1,0 +1,177 @@
+"""Benchmark of the traversal hot path on one MI355X.
+
+Workload (one "step" = one launch of the closest-hit kernel over the whole
+ray batch): 25 seeded displaced icospheres, subdivision 6 = 2,048,000
+triangles in the unit cube (the 2M-triangle point of the traversal workload;
+``--scene 64x7`` selects 64 x subdivision 7 = 20,971,520 triangles), and
+2^23 = 8,388,608 incoherent rays (origins uniform in the cube, directions
+uniform on the sphere), FP32, rays and BVH resident in HBM before timing.
+
+Prints one JSON line: value = closest-hit Grays/s; any-hit Grays/s, the
+roofline of the closest-hit kernel (algorithmic bytes per launch from the
+instrumented node/triangle fetch counts on a 1/64 ray subsample / measured
+kernel time, vs 8 TB/s) and the CPU baseline (the oracle's BVH2 traversal of
+the same rays on the host cores) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpaths/s + Grays/s; relMSE at equal time vs FP64 oracle, guided vs plain PT"
+HBM_PEAK_GBS = 8000.0
+RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 16, 128, 48
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--scene", default="25x6", help="COUNTxSUBDIV icospheres")
+    p.add_argument("--rays", type=int, default=1 << 23)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a plain 1 GiB device copy (PMC scale factor)")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(tag):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated), recorded in profiles/<tag>_pmc.md."""
+    path = os.path.join(ROOT, "profiles", f"{tag}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(tris, rays_np):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    bvh = ob.OracleBVH(tris)
+    sample = rays_np
+    bvh.closest(sample[:4096], nthreads)  # warm
+    t0 = time.perf_counter()
+    bvh.closest(sample, nthreads)
+    dt = time.perf_counter() - t0
+    return {"value": sample.shape[0] / dt / 1e9, "unit": "Grays/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"oracle BVH2 watertight closest-hit, all {sample.shape[0]} rays "
+                      f"of the workload, {nthreads} threads, {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are "
+                                   "not part of it (the path shards by rays/samples, "
+                                   "see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    import wayland_amd as wl
+    from wayland_amd import scenes
+
+    count, subdiv = (int(x) for x in args.scene.split("x"))
+    t0 = time.perf_counter()
+    sc = scenes.icosphere_scene(count, subdiv)
+    log(f"scene {count}x{subdiv}: {sc.num_triangles} tris, gen {time.perf_counter() - t0:.1f}s")
+    ctx = wl.Context(0)
+    t0 = time.perf_counter()
+    accel = wl.Accel(ctx, sc)
+    info = accel.info()
+    log(f"accel build {time.perf_counter() - t0:.1f}s: {info}")
+    rays_np = scenes.traversal_rays(args.rays, seed=2)
+    rays = torch.from_numpy(rays_np).cuda()
+    n = rays.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    bits = torch.empty(((n + 31) // 32,), dtype=torch.int32, device="cuda")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            fn()
+        end.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / 1e3 / steps
+
+    t_closest = timed(lambda: accel.trace_closest(rays, hits), args.steps, args.warmup)
+    log(f"closest: {t_closest * 1e3:.3f} ms/launch, {n / t_closest / 1e9:.3f} Grays/s")
+    t_any = timed(lambda: accel.trace_any(rays, bits), args.steps, args.warmup)
+    log(f"any: {t_any * 1e3:.3f} ms/launch, {n / t_any / 1e9:.3f} Grays/s")
+    hit_rate = float((hits[:, 1].view(torch.int32) != -1).float().mean().item())
+
+    # algorithmic bytes per ray from the instrumented kernel on a subsample
+    sub = rays[::64].contiguous()
+    cnt = torch.empty((sub.shape[0], 2), dtype=torch.int32, device="cuda")
+    accel.trace_closest(sub, None, cnt)
+    torch.cuda.synchronize()
+    c = cnt.double().mean(0).cpu().numpy()
+    bytes_per_ray = RAY_BYTES + HIT_BYTES + c[0] * NODE_BYTES + c[1] * TRI_BYTES
+    achieved = bytes_per_ray * n / t_closest / 1e9
+    tag = f"trace_closest_{count}x{subdiv}"
+    traffic = traffic_from_profiles(tag)
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sc.triangles(), rays_np)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": n / t_closest / 1e9,
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_closest * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"incoherent closest-hit traversal, {count} displaced "
+                               f"icospheres subdiv {subdiv} ({sc.num_triangles} tris), "
+                               f"{n} rays, origins U[0,1]^3, dirs uniform on S^2",
+                   "triangles": sc.num_triangles, "rays": n},
+        "anyhit_grays_per_s": n / t_any / 1e9,
+        "hit_rate": hit_rate,
+        "bvh": {"nodes": info["inner_nodes"], "depth": info["depth"],
+                "build_s": info["build_seconds"], "device_bytes": info["device_bytes"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "bytes_per_ray": bytes_per_ray, "nodes_per_ray": c[0],
+                     "tris_per_ray": c[1]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

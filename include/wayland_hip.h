@@ -1,0 +1,250 @@
+/*
+ * wayland_hip.h — C ABI of the MI355X (gfx950) rendering backend for
+ * EasyRender (Extra-Creativity/Wayland).
+ *
+ * This is the boundary that replaces the OptiX backend
+ * (src/Backend/Optix/Core/*) under the EasyRender path tracer.  Every entry
+ * point names the reference interface it stands in for.  Plain C: opaque
+ * handles, plain pointers and sizes; a stream is passed as `void *` holding a
+ * hipStream_t (NULL = the context's own stream).
+ *
+ * Conventions
+ *  - Every call returns a wl_status (0 = WL_SUCCESS) and never throws or
+ *    aborts across the boundary.  wl_last_error_message() gives detail for
+ *    the calling thread's last failure.  (Reference: ErrorCheck.h:9-58 only
+ *    reports errors in debug builds; here they are always reported.)
+ *  - Handles are owned by the library and freed by the matching *_destroy.
+ *    A handle is not internally synchronised: calls on one handle from
+ *    several threads need external locking.
+ *  - Ray, hit and film buffers are caller-owned DEVICE memory.  Geometry and
+ *    scene description inputs are HOST memory and are copied, so callers may
+ *    free them after the call (as TriangleDataBuffer copies its input,
+ *    BuildInput.cpp:118-148).
+ *  - Work is enqueued on the given stream; the call returns without
+ *    synchronising unless stated.
+ */
+#ifndef WAYLAND_HIP_H
+#define WAYLAND_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum wl_status
+{
+    WL_SUCCESS = 0,
+    WL_ERROR_INVALID_VALUE = 1,
+    WL_ERROR_OUT_OF_MEMORY = 2,
+    WL_ERROR_DEVICE = 3,          /* a HIP runtime call failed            */
+    WL_ERROR_NOT_SUPPORTED = 4,
+    WL_ERROR_LIMIT_EXCEEDED = 5,  /* e.g. BVH deeper than the stack       */
+    WL_ERROR_NO_DEVICE = 6,
+    WL_ERROR_UNKNOWN_PROGRAM = 7,
+    WL_ERROR_QUEUE_OVERFLOW = 8
+} wl_status;
+
+#define WL_INVALID_INDEX 0xFFFFFFFFu
+
+typedef struct wl_context_s *wl_context;
+typedef struct wl_accel_s *wl_accel;
+typedef struct wl_scene_s *wl_scene;
+
+/* ---- errors (reference: HostUtils/ErrorCheck.h:9-58, ErrorCheck.cpp:14-22) */
+const char *wl_status_string(wl_status status);
+const char *wl_last_error_message(void);
+
+/* ---- contexts (reference: Core/ContextManager.cpp:43-56 InitAndGetDeviceCount,
+ *      :99-145 CreateContext, ContextManager.h:84-123 LocalContextSetter) ---- */
+wl_status wl_device_count(int *count);
+/* One context = one device + one stream (default priority). */
+wl_status wl_context_create(int device, wl_context *out);
+wl_status wl_context_destroy(wl_context ctx);
+wl_status wl_context_get_stream(wl_context ctx, void **stream);
+wl_status wl_context_synchronize(wl_context ctx);
+/* ContextManager::SetCachePath (ContextManager.h:58): kernels are compiled
+ * ahead of time for gfx950, so the path is only recorded. */
+wl_status wl_set_cache_path(wl_context ctx, const char *path);
+
+/* ---- acceleration structures --------------------------------------------
+ * wl_tri_input mirrors one TriangleBuildInputArray::AddBuildInput call
+ * (BuildInput.cpp:193-207): FLOAT3 vertices, UINT3 indices, one geometry
+ * flag and `num_sbt_records` SBT records for this input.                   */
+typedef struct wl_tri_input
+{
+    const float *vertices;      /* host, 3 floats per vertex (12-byte stride) */
+    uint32_t num_vertices;
+    const uint32_t *indices;    /* host, 3 per triangle, input-local          */
+    uint32_t num_triangles;
+    uint32_t geometry_flags;    /* GeometryFlags (BuildInput.h:50-56)         */
+    uint32_t num_sbt_records;   /* 1 (one record per build input)            */
+} wl_tri_input;
+
+typedef struct wl_accel_info
+{
+    uint64_t num_triangles;
+    uint32_t num_inputs;
+    uint32_t num_nodes;         /* BVH8 node slots allocated                  */
+    uint32_t inner_nodes;       /* nodes in use                               */
+    uint32_t leaves;
+    uint32_t depth;             /* BVH8 levels (root = 1)                     */
+    uint32_t pad_;
+    uint64_t device_bytes;      /* nodes + triangle records on the device     */
+    double sah_cost;
+    double build_seconds;
+} wl_accel_info;
+
+/* StaticAccelStructure(buildInputs) (AccelStructure.cpp:133-162) → SAH BVH8.
+ * Global primitive id of triangle j of input i = (sum of num_triangles of
+ * inputs < i) + j. */
+wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
+                                   uint32_t num_inputs, uint32_t build_flags,
+                                   wl_accel *out);
+wl_status wl_accel_destroy(wl_accel accel);
+wl_status wl_accel_get_info(wl_accel accel, wl_accel_info *info);
+/* TriangleBuildInputArray::GetDepth() (BuildInput.h:192) — always 1. */
+wl_status wl_accel_depth(wl_accel accel, uint32_t *depth);
+/* Triangle offset of each build input (num_inputs entries). */
+wl_status wl_accel_input_offsets(wl_accel accel, uint32_t *offsets);
+
+/* ---- raw traversal (optixTrace call sites PathTracing.cu:72-75, :92-102)
+ * wl_ray is what optixTrace receives: origin, tmin, direction, tmax.      */
+typedef struct wl_ray
+{
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} wl_ray;
+
+/* Closest hit: t, global primitive id, barycentrics (b1, b2) weighting v1, v2
+ * as optixGetTriangleBarycentrics() does.  Miss: prim = WL_INVALID_INDEX,
+ * t = +inf.  Ties in t go to the smaller primitive id.  Watertight FP32 test
+ * (Woop, Benthin, Wald 2013); hits are accepted for tmin <= t <= tmax. */
+typedef struct wl_hit
+{
+    float t;
+    uint32_t prim;
+    float b1, b2;
+} wl_hit;
+
+wl_status wl_trace_closest(wl_accel accel, const wl_ray *d_rays, uint32_t n,
+                           wl_hit *d_hits, void *stream);
+/* Any hit with OPTIX_RAY_FLAG_TERMINATE_ON_FIRST_HIT semantics: bit (i % 32)
+ * of d_occluded[i / 32] is set when ray i hits anything in [tmin, tmax].
+ * d_occluded holds ceil(n / 32) words. */
+wl_status wl_trace_any(wl_accel accel, const wl_ray *d_rays, uint32_t n,
+                       uint32_t *d_occluded, void *stream);
+/* Instrumented closest hit: also writes, per ray, the number of BVH8 nodes
+ * fetched and triangles tested (d_counts[2*i], d_counts[2*i+1]). */
+wl_status wl_trace_closest_counted(wl_accel accel, const wl_ray *d_rays,
+                                   uint32_t n, wl_hit *d_hits,
+                                   uint32_t *d_counts, void *stream);
+
+/* ---- scene for the path tracer ---------------------------------------- */
+/* Device/Material.h:7-21 DisneyMaterial (64 bytes, last float padding). */
+typedef struct wl_disney_material
+{
+    float color[3];
+    float metallic, roughness, specular, specularTint, eta, trans, subsurface,
+        anisotropic, sheen, sheenTint, clearcoat, clearcoatgloss;
+    float pad_;
+} wl_disney_material;
+
+/* Device/Camera.h:9-15 PinholeCamFrame; `right` already aspect-scaled as
+ * PathTracingProgramManager.cpp:22-28 does. */
+typedef struct wl_pinhole_camera
+{
+    float pos[3];
+    float lookat[3];
+    float up[3];
+    float right[3];
+} wl_pinhole_camera;
+
+/* Per build input shading data: the HitData of PathTracingLaunchParams.h:64-78
+ * without device pointers (the library owns the copies). */
+typedef struct wl_mesh_shading
+{
+    const float *normals;       /* host, 3 floats per vertex, or NULL          */
+    wl_disney_material material;
+    uint32_t area_light;        /* index into lights, or WL_INVALID_INDEX      */
+    uint32_t pad_[3];
+} wl_mesh_shading;
+
+/* Device/Light.h:9-17 DeviceAreaLight: emitter = the triangles of `mesh`. */
+typedef struct wl_area_light
+{
+    uint32_t mesh;
+    float L[3];
+    uint32_t two_sided;
+} wl_area_light;
+
+wl_status wl_scene_create(wl_context ctx, wl_accel accel,
+                          const wl_mesh_shading *meshes, uint32_t num_meshes,
+                          const wl_area_light *lights, uint32_t num_lights,
+                          const wl_pinhole_camera *camera, const float bg[3],
+                          wl_scene *out);
+wl_status wl_scene_destroy(wl_scene scene);
+wl_status wl_scene_set_camera(wl_scene scene, const wl_pinhole_camera *camera);
+
+/* ---- wavefront path tracer (PathTracing.cu:40-291) --------------------- */
+enum
+{
+    WL_STRATEGY_UPT = 0,
+    WL_STRATEGY_NEE = 1,
+    WL_STRATEGY_MIS = 2
+};
+enum
+{
+    WL_PT_SORT_BY_MATERIAL = 1u << 0 /* shade queue binned by material id */
+};
+
+typedef struct wl_pt_config
+{
+    uint32_t width, height;
+    uint32_t spp;          /* samples per pixel in this call                  */
+    uint32_t frame_offset; /* sample s uses the reference's frame id offset+s */
+    uint32_t max_depth;    /* PathTracing.cu:235 (25)                          */
+    uint32_t rr_depth;     /* PathTracing.cu:130 RR when depth > rr_depth (5) */
+    float rr_prob;         /* PathTracing.cu:68 (0.9)                          */
+    float epsilon;         /* PathTracing.cu:19 (1e-3)                         */
+    uint32_t strategy;     /* PathTracing.cu:21-29 (MIS)                       */
+    uint32_t flags;
+    uint32_t max_batch_paths; /* 0 = library default                          */
+    uint32_t pad_;
+} wl_pt_config;
+
+typedef struct wl_pt_stats
+{
+    uint64_t paths;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+    uint32_t max_bounce;
+    uint32_t pad_;
+    double seconds;        /* host wall time of the call (synchronising)     */
+} wl_pt_stats;
+
+/* Adds cfg->spp samples per pixel into a compensated FP32 film:
+ * d_film = 6 floats per pixel (sum RGB, Kahan compensation RGB), caller
+ * zeroes it before the first call.  Mean = sum / total samples.  If stats is
+ * non-NULL the call synchronises and fills it. */
+wl_status wl_pt_render(wl_scene scene, const wl_pt_config *cfg, float *d_film,
+                       wl_pt_stats *stats, void *stream);
+/* The per-sample radiance of frame `frame` (one sample per pixel, TEA seed
+ * (pixel, frame)): d_rgb = W*H*3 floats.  For parity with the reference. */
+wl_status wl_pt_render_samples(wl_scene scene, const wl_pt_config *cfg,
+                               uint32_t frame, float *d_rgb, void *stream);
+/* One progressive frame exactly as the reference raygen ends
+ * (PathTracing.cu:145-158): running mean into d_radiance (RGBA float) with
+ * float factors and double temporaries, then ACES (Tonemap.h:34-42) -> u8. */
+wl_status wl_pt_launch_frame(wl_scene scene, const wl_pt_config *cfg,
+                             uint32_t frame, float *d_radiance_rgba,
+                             uint8_t *d_color_rgba, void *stream);
+/* Film resolve: d_rgb[i] = sum / nsamples (+ compensation). */
+wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
+                          uint32_t nsamples, float *d_rgb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

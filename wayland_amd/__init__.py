@@ -1,0 +1,252 @@
+"""wayland_amd — MI355X (gfx950) rendering backend for EasyRender.
+
+Python host-side mirror of the C ABI in include/wayland_hip.h, used by the
+tests and the benchmark.  Device buffers are torch tensors on a ROCm device
+(plumbing only); all work runs in the HIP kernels of libwayland_hip.so, on
+torch's current stream so that it orders with surrounding torch work.
+
+    ctx   = Context(0)                      # ContextManager
+    accel = Accel(ctx, scene)               # StaticAccelStructure
+    hits  = accel.trace_closest(rays)       # optixTrace, closest hit
+    occ   = accel.trace_any(rays)           # optixTrace, terminate on first hit
+    pt    = PathTracer(ctx, accel, scene)   # PathTracing program
+    film  = pt.render(cfg)                  # wavefront PT into a compensated film
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import WL_INVALID_INDEX, WlError  # noqa: F401
+
+__all__ = ["Context", "Accel", "PathTracer", "WlError", "WL_INVALID_INDEX",
+           "HIT_DTYPE", "pt_config"]
+
+HIT_DTYPE = np.dtype([("t", np.float32), ("prim", np.uint32),
+                      ("b1", np.float32), ("b2", np.float32)])
+
+STRATEGY_UPT, STRATEGY_NEE, STRATEGY_MIS = 0, 1, 2
+PT_SORT_BY_MATERIAL = 1
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(device=None):
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _lib.load().wl_device_count(C.byref(n))
+    return n.value
+
+
+class Context:
+    """One device + one stream (reference: ContextManager)."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        h = C.c_void_p()
+        _lib.call("wl_context_create", device, C.byref(h))
+        self.h = h
+
+    def synchronize(self):
+        _lib.call("wl_context_synchronize", self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Accel:
+    """Triangle acceleration structure over a scenes.Scene (one build input
+    per mesh), reference StaticAccelStructure + TriangleBuildInputArray."""
+
+    def __init__(self, ctx: Context, scene=None, meshes=None):
+        self.ctx = ctx
+        meshes = meshes if meshes is not None else scene.meshes
+        keep = []
+        arr = (_lib.TriInput * len(meshes))()
+        for i, m in enumerate(meshes):
+            v = np.ascontiguousarray(m.vertices, np.float32).reshape(-1, 3)
+            idx = np.ascontiguousarray(m.indices, np.uint32).reshape(-1, 3)
+            keep += [v, idx]
+            arr[i].vertices = v.ctypes.data
+            arr[i].num_vertices = v.shape[0]
+            arr[i].indices = idx.ctypes.data
+            arr[i].num_triangles = idx.shape[0]
+            arr[i].geometry_flags = 1
+            arr[i].num_sbt_records = 1
+        h = C.c_void_p()
+        _lib.call("wl_accel_build_triangles", ctx.h, arr, len(meshes), 0, C.byref(h))
+        self.h = h
+        self.num_inputs = len(meshes)
+
+    def info(self) -> dict:
+        inf = _lib.AccelInfo()
+        _lib.call("wl_accel_get_info", self.h, C.byref(inf))
+        return {k: getattr(inf, k) for k, _ in inf._fields_ if k != "pad_"}
+
+    def input_offsets(self) -> np.ndarray:
+        out = np.zeros(self.num_inputs, np.uint32)
+        _lib.call("wl_accel_input_offsets", self.h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out
+
+    # rays: torch float32 tensor (n, 8) on the device
+    def trace_closest(self, rays, hits=None, counts=None):
+        torch = _torch()
+        n = rays.shape[0]
+        assert rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1] == 8
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        assert hits.shape == (n, 4) and hits.is_contiguous()
+        if counts is not None:
+            assert counts.shape == (n, 2) and counts.dtype == torch.int32
+            _lib.call("wl_trace_closest_counted", self.h, C.c_void_p(rays.data_ptr()), n,
+                      C.c_void_p(hits.data_ptr()), C.c_void_p(counts.data_ptr()),
+                      _stream_ptr(rays.device))
+        else:
+            _lib.call("wl_trace_closest", self.h, C.c_void_p(rays.data_ptr()), n,
+                      C.c_void_p(hits.data_ptr()), _stream_ptr(rays.device))
+        return hits
+
+    def trace_any(self, rays, bits=None):
+        torch = _torch()
+        n = rays.shape[0]
+        assert rays.dtype == torch.float32 and rays.is_contiguous() and rays.shape[1] == 8
+        words = (n + 31) // 32
+        if bits is None:
+            bits = torch.empty(max(words, 1), dtype=torch.int32, device=rays.device)
+        assert bits.numel() >= words
+        _lib.call("wl_trace_any", self.h, C.c_void_p(rays.data_ptr()), n,
+                  C.c_void_p(bits.data_ptr()), _stream_ptr(rays.device))
+        return bits
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_accel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hits_to_numpy(hits) -> np.ndarray:
+    """(n, 4) float32 device tensor -> structured array with HIT_DTYPE."""
+    a = hits.detach().cpu().numpy()
+    return np.ascontiguousarray(a).view(HIT_DTYPE).reshape(-1)
+
+
+def unpack_bits(bits, n) -> np.ndarray:
+    w = bits.detach().cpu().numpy().view(np.uint32)
+    b = np.unpackbits(w.view(np.uint8), bitorder="little")
+    return b[:n].astype(np.uint8)
+
+
+def pt_config(width, height, spp=1, frame_offset=0, max_depth=25, rr_depth=5,
+              rr_prob=0.9, epsilon=1e-3, strategy=STRATEGY_MIS, flags=0,
+              max_batch_paths=0):
+    return _lib.PTConfig(width, height, spp, frame_offset, max_depth, rr_depth,
+                         rr_prob, epsilon, strategy, flags, max_batch_paths, 0)
+
+
+class PathTracer:
+    """The PathTracing program on a scene (reference:
+    src/Frontend/Programs/PathTracing/*)."""
+
+    def __init__(self, ctx: Context, accel: Accel, scene):
+        self.ctx, self.accel, self.scene = ctx, accel, scene
+        n = len(scene.meshes)
+        meshes = (_lib.MeshShading * n)()
+        keep = []
+        light_of = {}
+        lights = []
+        for i, m in enumerate(scene.meshes):
+            if m.light_L is not None:
+                light_of[i] = len(lights)
+                lights.append((i, m.light_L, m.two_sided))
+        for i, m in enumerate(scene.meshes):
+            if m.normals is not None:
+                nrm = np.ascontiguousarray(m.normals, np.float32).reshape(-1, 3)
+                keep.append(nrm)
+                meshes[i].normals = nrm.ctypes.data
+            else:
+                meshes[i].normals = None
+            meshes[i].material.values[:] = [float(x) for x in np.asarray(m.material, np.float32)]
+            meshes[i].area_light = light_of.get(i, WL_INVALID_INDEX)
+        larr = (_lib.AreaLight * max(1, len(lights)))()
+        for k, (mi, L, two) in enumerate(lights):
+            larr[k].mesh = mi
+            larr[k].L[:] = [float(x) for x in L]
+            larr[k].two_sided = 1 if two else 0
+        cam = _lib.PinholeCamera()
+        if scene.camera is not None:
+            cam.pos[:] = [float(x) for x in scene.camera.pos]
+            cam.lookat[:] = [float(x) for x in scene.camera.lookat]
+            cam.up[:] = [float(x) for x in scene.camera.up]
+            cam.right[:] = [float(x) for x in scene.camera.right]
+        bg = (C.c_float * 3)(*[float(x) for x in scene.bg])
+        h = C.c_void_p()
+        _lib.call("wl_scene_create", ctx.h, accel.h, meshes, n, larr, len(lights),
+                  C.byref(cam), bg, C.byref(h))
+        self.h = h
+
+    def render(self, cfg, film=None, stats=True):
+        torch = _torch()
+        npix = cfg.width * cfg.height
+        if film is None:
+            film = torch.zeros((npix, 6), dtype=torch.float32, device="cuda")
+        st = _lib.PTStats()
+        _lib.call("wl_pt_render", self.h, C.byref(cfg), C.c_void_p(film.data_ptr()),
+                  C.byref(st) if stats else None, _stream_ptr(film.device))
+        return film, ({k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
+                      if stats else None)
+
+    def render_samples(self, cfg, frame, out=None):
+        torch = _torch()
+        if out is None:
+            out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device="cuda")
+        _lib.call("wl_pt_render_samples", self.h, C.byref(cfg), frame,
+                  C.c_void_p(out.data_ptr()), _stream_ptr(out.device))
+        return out
+
+    def launch_frame(self, cfg, frame, radiance_rgba, color_rgba):
+        _lib.call("wl_pt_launch_frame", self.h, C.byref(cfg), frame,
+                  C.c_void_p(radiance_rgba.data_ptr()), C.c_void_p(color_rgba.data_ptr()),
+                  _stream_ptr(radiance_rgba.device))
+
+    def resolve(self, film, nsamples, out=None):
+        torch = _torch()
+        npix = film.shape[0]
+        if out is None:
+            out = torch.empty((npix, 3), dtype=torch.float32, device=film.device)
+        _lib.call("wl_film_resolve", self.ctx.h, C.c_void_p(film.data_ptr()), npix,
+                  nsamples, C.c_void_p(out.data_ptr()), _stream_ptr(film.device))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

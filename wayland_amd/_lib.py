@@ -1,0 +1,138 @@
+"""ctypes binding of libwayland_hip.so (the C ABI in include/wayland_hip.h).
+
+Loading fails loudly if the library has not been built: there is no CPU
+fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libwayland_hip.so")
+
+WL_INVALID_INDEX = 0xFFFFFFFF
+
+
+class WlError(RuntimeError):
+    def __init__(self, status, fn, message):
+        self.status = status
+        super().__init__(f"{fn} failed: {status_name(status)}: {message}")
+
+
+class TriInput(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint32),
+                ("indices", C.c_void_p), ("num_triangles", C.c_uint32),
+                ("geometry_flags", C.c_uint32), ("num_sbt_records", C.c_uint32)]
+
+
+class AccelInfo(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint64), ("num_inputs", C.c_uint32),
+                ("num_nodes", C.c_uint32), ("inner_nodes", C.c_uint32),
+                ("leaves", C.c_uint32), ("depth", C.c_uint32), ("pad_", C.c_uint32),
+                ("device_bytes", C.c_uint64), ("sah_cost", C.c_double),
+                ("build_seconds", C.c_double)]
+
+
+class DisneyMaterial(C.Structure):
+    _fields_ = [("values", C.c_float * 16)]
+
+
+class PinholeCamera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("lookat", C.c_float * 3),
+                ("up", C.c_float * 3), ("right", C.c_float * 3)]
+
+
+class MeshShading(C.Structure):
+    _fields_ = [("normals", C.c_void_p), ("material", DisneyMaterial),
+                ("area_light", C.c_uint32), ("pad_", C.c_uint32 * 3)]
+
+
+class AreaLight(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("L", C.c_float * 3), ("two_sided", C.c_uint32)]
+
+
+class PTConfig(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("frame_offset", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("rr_prob", C.c_float), ("epsilon", C.c_float),
+                ("strategy", C.c_uint32), ("flags", C.c_uint32),
+                ("max_batch_paths", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class PTStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("max_bounce", C.c_uint32),
+                ("pad_", C.c_uint32), ("seconds", C.c_double)]
+
+
+# name -> (restype, argtypes); every function declared in include/wayland_hip.h
+_VP = C.c_void_p
+_U32 = C.c_uint32
+SIGNATURES = {
+    "wl_status_string": (C.c_char_p, [C.c_int]),
+    "wl_last_error_message": (C.c_char_p, []),
+    "wl_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "wl_context_create": (C.c_int, [C.c_int, C.POINTER(_VP)]),
+    "wl_context_destroy": (C.c_int, [_VP]),
+    "wl_context_get_stream": (C.c_int, [_VP, C.POINTER(_VP)]),
+    "wl_context_synchronize": (C.c_int, [_VP]),
+    "wl_set_cache_path": (C.c_int, [_VP, C.c_char_p]),
+    "wl_accel_build_triangles": (C.c_int, [_VP, C.POINTER(TriInput), _U32, _U32, C.POINTER(_VP)]),
+    "wl_accel_destroy": (C.c_int, [_VP]),
+    "wl_accel_get_info": (C.c_int, [_VP, C.POINTER(AccelInfo)]),
+    "wl_accel_depth": (C.c_int, [_VP, C.POINTER(_U32)]),
+    "wl_accel_input_offsets": (C.c_int, [_VP, C.POINTER(_U32)]),
+    "wl_trace_closest": (C.c_int, [_VP, _VP, _U32, _VP, _VP]),
+    "wl_trace_any": (C.c_int, [_VP, _VP, _U32, _VP, _VP]),
+    "wl_trace_closest_counted": (C.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
+    "wl_scene_create": (C.c_int, [_VP, _VP, C.POINTER(MeshShading), _U32,
+                                  C.POINTER(AreaLight), _U32, C.POINTER(PinholeCamera),
+                                  C.POINTER(C.c_float), C.POINTER(_VP)]),
+    "wl_scene_destroy": (C.c_int, [_VP]),
+    "wl_scene_set_camera": (C.c_int, [_VP, C.POINTER(PinholeCamera)]),
+    "wl_pt_render": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, C.POINTER(PTStats), _VP]),
+    "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
+    "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),
+    "wl_film_resolve": (C.c_int, [_VP, _VP, _U32, _U32, _VP, _VP]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (raises if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                          "or python wayland_amd/build.py")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def status_name(s):
+    try:
+        return load().wl_status_string(s).decode()
+    except Exception:
+        return str(s)
+
+
+def check(status, fn):
+    if status != 0:
+        msg = load().wl_last_error_message()
+        raise WlError(status, fn, msg.decode() if msg else "")
+    return status
+
+
+def call(name, *args):
+    fn = getattr(load(), name)
+    return check(fn(*args), name)

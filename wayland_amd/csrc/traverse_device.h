@@ -1,0 +1,335 @@
+// traverse_device.h — gfx950 BVH8 traversal, one ray per lane.
+//
+// Watertight FP32 ray/triangle test (Woop, Benthin, Wald, JCGT 2013) with the
+// same operation order as the host oracle: this translation unit is compiled
+// with -ffp-contract=off, so every + - * / below rounds separately and the
+// division and square root are IEEE correctly rounded.  Box tests use
+// explicit fmaf (only the triangle test must match the host bit for bit: the
+// box test decides what is visited, never what is hit, because boxes are
+// conservative).
+//
+// Traversal state per lane: a current node group (child_base, hit mask in
+// key space), a triangle mask of the node just visited, and an LDS stack of
+// 4-byte group entries, WL_BVH8_STACK_DEPTH deep, laid out [depth][lane] so
+// the 64 lanes of a wave touch 64 consecutive words.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh8.h"
+
+#define WL_INVALID 0xFFFFFFFFu
+
+// Compiled for the device by hipcc and, for the CPU emulation used by the
+// tests (hostemu.cpp), for the host.  WL_TRAV_CHECK is empty on the device;
+// the host emulation defines it to record out-of-range accesses.
+#define WL_HD __host__ __device__ __forceinline__
+#ifndef WL_TRAV_CHECK
+#define WL_TRAV_CHECK(cond, what) ((void)0)
+#endif
+
+namespace wl
+{
+
+WL_HD float wl_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+WL_HD uint32_t wl_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+
+struct TravRay
+{
+    float o[3];
+    float tmin, tmax;
+    float inv[3];
+    float Sx, Sy, Sz;
+    float okx, oky, okz;   // origin permuted by (kx, ky, kz)
+    int kx, ky, kz;
+    uint32_t xkey;         // slot ^ xkey = visit order key (near first)
+    uint32_t neg;          // bit k: d_k < 0
+};
+
+WL_HD float sel3(int k, float a, float b, float c)
+{
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+
+WL_HD void trav_setup(TravRay &r, float ox, float oy,
+                                           float oz, float dx, float dy,
+                                           float dz, float tmin, float tmax)
+{
+    r.o[0] = ox;
+    r.o[1] = oy;
+    r.o[2] = oz;
+    r.tmin = tmin;
+    r.tmax = tmax;
+    float d[3] = { dx, dy, dz };
+    int kz = 0;
+    if (fabsf(d[1]) > fabsf(d[kz]))
+        kz = 1;
+    if (fabsf(d[2]) > fabsf(d[kz]))
+        kz = 2;
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dkz = sel3(kz, dx, dy, dz);
+    if (dkz < 0.0f)
+    {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.Sx = sel3(kx, dx, dy, dz) / dkz;
+    r.Sy = sel3(ky, dx, dy, dz) / dkz;
+    r.Sz = 1.0f / dkz;
+    r.okx = sel3(kx, ox, oy, oz);
+    r.oky = sel3(ky, ox, oy, oz);
+    r.okz = sel3(kz, ox, oy, oz);
+    uint32_t neg = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        float di = d[i];
+        if (fabsf(di) < 1e-30f)
+            di = copysignf(1e-30f, di);
+        r.inv[i] = 1.0f / di;
+        if (r.inv[i] < 0.0f)
+            neg |= 1u << i;
+    }
+    r.neg = neg;
+    r.xkey = neg ^ 7u;
+}
+
+// Watertight test; returns true and (t, b1, b2) for tmin <= t <= tmax.
+WL_HD bool tri_test(const TravRay &r, const float4 w0,
+                                         const float4 w1, const float4 w2,
+                                         float &t_out, float &b1_out,
+                                         float &b2_out)
+{
+    // A = v0 - o, B = v1 - o, C = v2 - o, components permuted (kx, ky, kz)
+    const float Akx = sel3(r.kx, w0.x, w0.y, w0.z) - r.okx;
+    const float Aky = sel3(r.ky, w0.x, w0.y, w0.z) - r.oky;
+    const float Akz = sel3(r.kz, w0.x, w0.y, w0.z) - r.okz;
+    const float Bkx = sel3(r.kx, w0.w, w1.x, w1.y) - r.okx;
+    const float Bky = sel3(r.ky, w0.w, w1.x, w1.y) - r.oky;
+    const float Bkz = sel3(r.kz, w0.w, w1.x, w1.y) - r.okz;
+    const float Ckx = sel3(r.kx, w1.z, w1.w, w2.x) - r.okx;
+    const float Cky = sel3(r.ky, w1.z, w1.w, w2.x) - r.oky;
+    const float Ckz = sel3(r.kz, w1.z, w1.w, w2.x) - r.okz;
+    const float Ax = Akx - r.Sx * Akz;
+    const float Ay = Aky - r.Sy * Akz;
+    const float Bx = Bkx - r.Sx * Bkz;
+    const float By = Bky - r.Sy * Bkz;
+    const float Cx = Ckx - r.Sx * Ckz;
+    const float Cy = Cky - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f)
+    {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f))
+        return false;
+    const float det = U + V + W;
+    if (det == 0.0f)
+        return false;
+    const float Az = r.Sz * Akz;
+    const float Bz = r.Sz * Bkz;
+    const float Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float rcp = 1.0f / det;
+    const float t = T * rcp;
+    if (!(t >= r.tmin && t <= r.tmax))
+        return false;
+    t_out = t;
+    b1_out = V * rcp;
+    b2_out = W * rcp;
+    return true;
+}
+
+WL_HD float q_lo16(uint32_t w) { return (float)(w & 0xFFFFu); }
+WL_HD float q_hi16(uint32_t w) { return (float)(w >> 16); }
+
+struct TravResult
+{
+    float t;
+    uint32_t prim;
+    float b1, b2;
+};
+
+// Traverse one ray.  ANY: stop at the first hit.  Counts are optional
+// instrumentation (nodes fetched, triangles tested).
+template <bool ANY, bool COUNT, int BLOCK>
+WL_HD TravResult
+trav_run(const WlBvh8Node *__restrict__ nodes,
+         const WlTriRecord *__restrict__ tris, const TravRay &r, bool active,
+         uint32_t *lds_stack_lane, uint32_t &n_nodes, uint32_t &n_tris)
+{
+    TravResult res;
+    res.t = r.tmax;
+    res.prim = WL_INVALID;
+    res.b1 = 0.0f;
+    res.b2 = 0.0f;
+    int sp = 0;
+    uint32_t grp_base = 0, grp_mask = 0;
+    uint32_t node = 0;
+    bool visit = active;
+    const float tmin = r.tmin;
+    for (;;)
+    {
+        uint32_t trimask = 0, tri_base = 0;
+        if (visit)
+        {
+            WL_TRAV_CHECK(node < wl_trav_nodes_bound, "node index");
+            const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
+            const uint4 h0 = np[0];
+            const uint4 h1 = np[1];
+            const uint4 qlx = np[2], qly = np[3], qlz = np[4];
+            const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
+            if (COUNT)
+                ++n_nodes;
+            const float px = wl_u2f(h0.x), py = wl_u2f(h0.y),
+                        pz = wl_u2f(h0.z);
+            const uint32_t ew = h0.w;
+            const float sx = wl_u2f((ew & 0xFFu) << 23);
+            const float sy = wl_u2f(((ew >> 8) & 0xFFu) << 23);
+            const float sz = wl_u2f(((ew >> 16) & 0xFFu) << 23);
+            const uint32_t imask = ew >> 24;
+            const uint32_t child_base = h1.x;
+            tri_base = h1.y;
+            const uint32_t meta_lo = h1.z, meta_hi = h1.w;
+            // per-axis plane parameters: t = q * sinv + adj
+            const float ax = (px - r.o[0]) * r.inv[0];
+            const float ay = (py - r.o[1]) * r.inv[1];
+            const float az = (pz - r.o[2]) * r.inv[2];
+            const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
+            // near/far quantised planes chosen by the ray's direction signs
+            const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+            const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
+            const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
+            const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
+            const float cull = res.t * 1.00001f;
+            const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
+            const uint32_t fxw[4] = { fqx.x, fqx.y, fqx.z, fqx.w };
+            const uint32_t nyw[4] = { nqy.x, nqy.y, nqy.z, nqy.w };
+            const uint32_t fyw[4] = { fqy.x, fqy.y, fqy.z, fqy.w };
+            const uint32_t nzw[4] = { nqz.x, nqz.y, nqz.z, nqz.w };
+            const uint32_t fzw[4] = { fqz.x, fqz.y, fqz.z, fqz.w };
+            uint32_t inner_hits = 0;
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+            {
+                const int wi = s >> 1;
+                float qnx, qfx, qny, qfy, qnz, qfz;
+                if (s & 1)
+                {
+                    qnx = q_hi16(nxw[wi]); qfx = q_hi16(fxw[wi]);
+                    qny = q_hi16(nyw[wi]); qfy = q_hi16(fyw[wi]);
+                    qnz = q_hi16(nzw[wi]); qfz = q_hi16(fzw[wi]);
+                }
+                else
+                {
+                    qnx = q_lo16(nxw[wi]); qfx = q_lo16(fxw[wi]);
+                    qny = q_lo16(nyw[wi]); qfy = q_lo16(fyw[wi]);
+                    qnz = q_lo16(nzw[wi]); qfz = q_lo16(fzw[wi]);
+                }
+                const float tnx = __builtin_fmaf(qnx, six, ax);
+                const float tfx = __builtin_fmaf(qfx, six, ax);
+                const float tny = __builtin_fmaf(qny, siy, ay);
+                const float tfy = __builtin_fmaf(qfy, siy, ay);
+                const float tnz = __builtin_fmaf(qnz, siz, az);
+                const float tfz = __builtin_fmaf(qfz, siz, az);
+                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                float tf = fminf(fminf(tfx, tfy), tfz);
+                tf = __builtin_fmaf(fabsf(tf), 4e-6f, tf);
+                const bool hit = tn <= fminf(tf, cull);
+                if (hit)
+                {
+                    if (imask & (1u << s))
+                    {
+                        inner_hits |= 1u << (s ^ r.xkey);
+                    }
+                    else
+                    {
+                        const uint32_t m = ((s < 4 ? meta_lo : meta_hi) >> (8 * (s & 3))) & 0xFFu;
+                        // 0xFF marks an empty slot (its inverted box can
+                        // still pass the padded test on a tiny node)
+                        if (m != 0xFFu)
+                        {
+                            const uint32_t off = m >> 2, cnt = (m & 3u) + 1u;
+                            trimask |= ((1u << cnt) - 1u) << off;
+                        }
+                    }
+                }
+            }
+            grp_base = child_base;
+            grp_mask = inner_hits;
+            visit = false;
+        }
+        // ---- triangles of the node just visited
+        bool done = false;
+        while (trimask)
+        {
+            const uint32_t i = __builtin_ctz(trimask);
+            trimask &= trimask - 1u;
+            WL_TRAV_CHECK((size_t)tri_base + i < wl_trav_tris_bound, "triangle index");
+            const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
+            const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+            if (COUNT)
+                ++n_tris;
+            float t, b1, b2;
+            if (tri_test(r, w0, w1, w2, t, b1, b2))
+            {
+                const uint32_t prim = wl_f2u(w2.y);
+                if (ANY)
+                {
+                    res.t = t;
+                    res.prim = prim;
+                    res.b1 = b1;
+                    res.b2 = b2;
+                    done = true;
+                    break;
+                }
+                if (t < res.t || (t == res.t && prim < res.prim))
+                {
+                    res.t = t;
+                    res.prim = prim;
+                    res.b1 = b1;
+                    res.b2 = b2;
+                }
+            }
+        }
+        if (ANY && done)
+            break;
+        // ---- next node: from the current group, else from the stack
+        if (grp_mask == 0)
+        {
+            if (sp == 0)
+                break;
+            --sp;
+            const uint32_t e = lds_stack_lane[sp * BLOCK];
+            grp_base = (e >> 8) << 3;
+            grp_mask = e & 0xFFu;
+        }
+        const uint32_t pos = __builtin_ctz(grp_mask);
+        grp_mask &= grp_mask - 1u;
+        node = grp_base + (pos ^ r.xkey);
+        if (grp_mask)
+        {
+            WL_TRAV_CHECK(sp < WL_BVH8_STACK_DEPTH, "stack depth");
+            lds_stack_lane[sp * BLOCK] = ((grp_base >> 3) << 8) | grp_mask;
+            ++sp;
+        }
+        visit = true;
+    }
+    if (res.prim == WL_INVALID)
+        res.t = wl_u2f(0x7F800000u); // +inf
+    return res;
+}
+
+} // namespace wl
