@@ -42,7 +42,45 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (PMC scale factor)")
+    p.add_argument("--pt-spp", type=int, default=64,
+                   help="wavefront PT 1920x1080 at this spp on the same geometry "
+                        "(0 = skip)")
     return p.parse_args()
+
+
+def bench_pt(args, ctx, accel, sc, wl, torch):
+    """Wavefront PT workload: 1920x1080, depth 16, RR after bounce 4, on the
+    same icosphere geometry with per-object Disney materials, two quad
+    lights and a constant environment 0.1."""
+    from wayland_amd import scenes
+    count, subdiv = (int(x) for x in args.scene.split("x"))
+    W, H = 1920, 1080
+    wsc = scenes.wavefront_scene(count, subdiv, width=W, height=H)
+    waccel = wl.Accel(ctx, wsc)
+    pt = wl.PathTracer(ctx, waccel, wsc)
+    film = torch.zeros((W * H, 6), dtype=torch.float32, device="cuda")
+    warm = wl.pt_config(W, H, spp=1, max_depth=16, rr_depth=4)
+    pt.render(warm, film)
+    torch.cuda.synchronize()
+    film.zero_()
+    cfg = wl.pt_config(W, H, spp=args.pt_spp, max_depth=16, rr_depth=4)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    _, st = pt.render(cfg, film, stats=False)
+    end.record()
+    torch.cuda.synchronize()
+    dt = start.elapsed_time(end) / 1e3
+    _, st = pt.render(wl.pt_config(W, H, spp=1, max_depth=16, rr_depth=4,
+                                   frame_offset=10_000), None, stats=True)
+    rays_per_path = (st["extension_rays"] + st["shadow_rays"]) / st["paths"]
+    paths = W * H * args.pt_spp
+    out = {"workload": f"wavefront PT {W}x{H} {args.pt_spp} spp depth 16 RR>4, "
+                       f"{wsc.num_triangles} tris, 2 quad lights + env 0.1",
+           "mpaths_per_s": paths / dt / 1e6,
+           "grays_per_s": paths * rays_per_path / dt / 1e9,
+           "rays_per_path": rays_per_path, "seconds": dt}
+    log(f"pt: {out}")
+    return out
 
 
 def log(*a):
@@ -137,6 +175,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
+    pt_out = None
+    if args.pt_spp > 0:
+        pt_out = bench_pt(args, ctx, accel, sc, wl, torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc.triangles(), rays_np)
@@ -169,6 +211,7 @@ def main():
                      "bytes_per_ray": bytes_per_ray, "nodes_per_ray": c[0],
                      "tris_per_ray": c[1]},
         "cpu_baseline": cpu,
+        "pathtracer": pt_out,
     }
     print(json.dumps(out), flush=True)
 

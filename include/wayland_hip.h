@@ -6,7 +6,8 @@
  * (src/Backend/Optix/Core/*) under the EasyRender path tracer.  Every entry
  * point names the reference interface it stands in for.  Plain C: opaque
  * handles, plain pointers and sizes; a stream is passed as `void *` holding a
- * hipStream_t (NULL = the context's own stream).
+ * hipStream_t (NULL = HIP's null stream, as everywhere in HIP; the context's
+ * own stream is returned by wl_context_get_stream).
  *
  * Conventions
  *  - Every call returns a wl_status (0 = WL_SUCCESS) and never throws or

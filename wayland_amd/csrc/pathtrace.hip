@@ -1,0 +1,509 @@
+// pathtrace.hip — wavefront path tracer for gfx950.
+//
+// Restates the per-pixel megakernel of the reference (raygen loop
+// PathTracing.cu:40-139, __closesthit__radiance :181-289, __miss__radiance
+// :163-171, NEE :81-128, RR :130-138) as queue kernels:
+//
+//   raygen  -> [extend -> shade -> shadow] x max_depth -> accumulate
+//
+//   raygen     TEA seed (pixel, frame), pinhole camera ray, path state.
+//   extend     closest-hit BVH8 traversal of the active ray queue.
+//   shade      miss / emitter hit with MIS / depth cap / Disney BSDF
+//              sample-pdf-eval / NEE light sample + BSDF eval / RR, in the
+//              reference's RNG draw order; appends the continuing paths and
+//              the shadow rays to compacted queues (wave64 ballot + mbcnt
+//              prefix, one atomic per wave).
+//   shadow     any-hit traversal of the shadow queue; adds the precomputed
+//              NEE contribution of unoccluded rays.
+//   accumulate compensated (Kahan) FP32 film: sum + compensation per pixel.
+//
+// Radiance contributions reach a path in the reference's order (one per
+// bounce at most), so per-sample radiance matches a serial evaluation.
+// Every kernel loops over a device-side queue count with a fixed grid, so the
+// host enqueues a whole batch without synchronising.
+#include "shading_device.h"
+#include "traverse_device.h"
+#include "wl_pt.h"
+
+namespace wl
+{
+
+constexpr int kPtBlock = 256;
+
+// ---- wave-level append: ballot + mbcnt prefix, one atomic per wave --------
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0)
+        return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t prefix = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader)
+        base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + prefix;
+}
+
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ V3 v3of(float4 a) { return mk(a.x, a.y, a.z); }
+
+// ---- raygen -------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
+{
+    const uint32_t npaths = P.npix * P.nsamples;
+    for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < npaths;
+         p += gridDim.x * kPtBlock)
+    {
+        const uint32_t pixel = P.pix0 + p % P.npix;
+        const uint32_t s = p / P.npix;
+        const uint32_t x = pixel % P.width, y = pixel / P.width;
+        uint32_t seed = tea4(pixel, P.frame0 + s);
+        // PinholeGenerateRay (Camera.h:28-35): two draws, u then v
+        const float u = 2.f * ((float)(int)x + rnd(seed)) / (float)(int)P.width - 1.f;
+        const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
+        const V3 d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
+        const V3 o = ld3(P.cam.pos);
+        B.pos[p] = f4(o, 1.0f);                          // w: lastPdf
+        B.dir[p] = f4(d, __uint_as_float(seed));         // w: seed
+        B.thr[p] = f4(mk(1, 1, 1), __uint_as_float(0u)); // w: depth
+        B.ltt[p] = f4(mk(1, 1, 1), 0.0f);                // lastTraceTerm
+        B.rad[p] = f4(mk(0, 0, 0), 0.0f);
+        B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
+        B.qpath0[p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        B.counters[0] = npaths;
+}
+
+// ---- extend: closest hit for the active queue -----------------------------
+__global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int parity)
+{
+    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kPtBlock];
+    const uint32_t count = B.counters[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        B.counters[parity ^ 1] = 0; // next queue
+        B.counters[2] = 0;          // shadow queue
+        B.stats[0] += count;        // extension rays (single writer)
+    }
+    const wl_ray *rays = parity ? B.rays1 : B.rays0;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
+    {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < count;
+        TravRay r;
+        if (active)
+        {
+            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+            const float4 a = rp[0], b = rp[1];
+            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+        }
+        else
+            trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
+        uint32_t nn = 0, nt = 0;
+        TravResult h = trav_run<false, false, kPtBlock>(S.nodes, S.tris, r, active,
+                                                       stack + threadIdx.x, nn, nt);
+        if (active)
+            reinterpret_cast<float4 *>(B.hits)[i] =
+                make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+    }
+}
+
+// ---- shade ------------------------------------------------------------------
+__global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuffers B,
+                                                    int parity)
+{
+    const uint32_t count = B.counters[parity];
+    const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
+    uint32_t *qout = parity ? B.qpath0 : B.qpath1;
+    wl_ray *rout = parity ? B.rays0 : B.rays1;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
+    {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        uint32_t path = 0;
+        wl_ray next_ray{}, shadow_ray{};
+        V3 contrib = mk(0, 0, 0);
+        if (i < count)
+        {
+            path = qin[i];
+            const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
+            const float4 s_pos = B.pos[path], s_dir = B.dir[path], s_thr = B.thr[path];
+            const float4 s_ltt = B.ltt[path];
+            float4 s_rad = B.rad[path];
+            V3 rayPos = v3of(s_pos);
+            float lastPdf = s_pos.w;
+            V3 rayDir = v3of(s_dir);
+            uint32_t seed = __float_as_uint(s_dir.w);
+            V3 throughput = v3of(s_thr);
+            uint32_t depth = __float_as_uint(s_thr.w);
+            V3 lastTraceTerm = v3of(s_ltt);
+            V3 radiance = v3of(s_rad);
+            const uint32_t prim = __float_as_uint(hit.y);
+            bool done = false;
+            if (prim == WL_INVALID)
+            {
+                // __miss__radiance: radiance += throughput * bg
+                radiance = radiance + throughput * ld3(P.bg);
+                done = true;
+            }
+            else
+            {
+                depth += 1;
+                const float t = hit.x, b1 = hit.z, b2 = hit.w;
+                const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
+                const float4 w0 = sr[0], w1 = sr[1], w2 = sr[2];
+                const V3 va = mk(w0.x, w0.y, w0.z), vb = mk(w0.w, w1.x, w1.y),
+                         vc = mk(w1.z, w1.w, w2.x);
+                const uint32_t mesh = __float_as_uint(w2.y);
+                const PtMesh M = S.meshes[mesh];
+                V3 Ng = normalize(cross(vb - va, vc - va));
+                V3 Ns;
+                if (M.has_normal)
+                {
+                    const float4 *nr = reinterpret_cast<const float4 *>(S.normals) + 3 * (size_t)prim;
+                    const float4 n0 = nr[0], n1 = nr[1], n2 = nr[2];
+                    const V3 na = mk(n0.x, n0.y, n0.z), nb = mk(n0.w, n1.x, n1.y),
+                             nc = mk(n1.z, n1.w, n2.x);
+                    Ns = normalize(na * (1 - b1 - b2) + nb * b1 + nc * b2);
+                }
+                else
+                    Ns = Ng;
+                const V3 hitPos = rayPos + rayDir * t;
+                if (M.light != WL_INVALID)
+                {
+                    const PtLight lt = S.lights[M.light];
+                    if (depth == 1 || (P.strategy != WL_STRATEGY_NEE &&
+                                       (lt.two_sided || dot(Ns, rayDir) < 0)))
+                    {
+                        float neePdf = 0.f;
+                        if (P.strategy != WL_STRATEGY_UPT && depth > 1)
+                        {
+                            // PdfAreaLightPos (Pdf.h:20-31)
+                            float pdf = 1.0f / (float)P.num_lights;
+                            pdf /= (float)lt.tri_count;
+                            pdf /= tri_area(va, vb, vc);
+                            const float dist = t;
+                            neePdf = pdf * dist * dist / fabsf(dot(Ns, rayDir));
+                        }
+                        const float uptPdf = lastPdf;
+                        radiance = radiance + throughput * ld3(lt.L) * lastTraceTerm *
+                                                  (uptPdf / (uptPdf + neePdf));
+                    }
+                    done = true;
+                }
+                else if (depth >= P.max_depth)
+                {
+                    throughput = mk(0, 0, 0);
+                    done = true;
+                }
+                else
+                {
+                    const Mat mat = load_mat(S.materials + M.material);
+                    if (mat.trans < 0.1f)
+                    {
+                        if (dot(Ng, rayDir) > 0)
+                            Ng = -Ng;
+                        if (dot(Ns, rayDir) > 0)
+                            Ns = -Ns;
+                    }
+                    V3 nd = sample_disney(mat, Ns, -rayDir, seed);
+                    nd = normalize(nd);
+                    const float pdf = pdf_disney(mat, Ns, -rayDir, nd);
+                    if (pdf < 0.0f || __builtin_isnan(pdf))
+                    {
+                        done = true;
+                    }
+                    else
+                    {
+                        throughput = throughput * lastTraceTerm;
+                        const float cosWeight = dot(nd, Ns);
+                        V3 bsdf = eval_disney(mat, Ns, Ng, -rayDir, nd, mk(1, 1, 1));
+                        bsdf = clamp3(bsdf, 0.f, 1e30f);
+                        lastTraceTerm = bsdf * fabsf(cosWeight) / pdf;
+                        const V3 lastRayDir = rayDir;
+                        rayPos = hitPos;
+                        rayDir = nd;
+                        lastPdf = pdf;
+                        // ---- NEE at this vertex (PathTracing.cu:81-128)
+                        if (P.strategy != WL_STRATEGY_UPT && P.num_lights > 0)
+                        {
+                            uint32_t lid = (uint32_t)((float)P.num_lights * rnd(seed));
+                            lid = lid < P.num_lights ? lid : P.num_lights - 1;
+                            const PtLight lt = S.lights[lid];
+                            float lpdf = 1.0f / (float)P.num_lights;
+                            uint32_t triIdx = (uint32_t)(rnd(seed) * (float)lt.tri_count);
+                            triIdx = triIdx < lt.tri_count ? triIdx : lt.tri_count - 1;
+                            lpdf = lpdf / (float)lt.tri_count;
+                            const uint32_t lprim = lt.tri_offset + triIdx;
+                            const float4 *lr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)lprim;
+                            const float4 l0 = lr[0], l1 = lr[1], l2 = lr[2];
+                            const V3 p0 = mk(l0.x, l0.y, l0.z), p1 = mk(l0.w, l1.x, l1.y),
+                                     p2 = mk(l1.z, l1.w, l2.x);
+                            float u = rnd(seed), v = rnd(seed);
+                            if (u + v > 1)
+                            {
+                                u = 1 - u;
+                                v = 1 - v;
+                            }
+                            const float cx = 1 - u - v, cy = u;
+                            const V3 lpos = p0 + u * (p1 - p0) + v * (p2 - p0);
+                            lpdf = lpdf / tri_area(p0, p1, p2);
+                            V3 lN;
+                            if (lt.has_normal)
+                            {
+                                const float4 *nr = reinterpret_cast<const float4 *>(S.normals) + 3 * (size_t)lprim;
+                                const float4 n0 = nr[0], n1 = nr[1], n2 = nr[2];
+                                const V3 na = mk(n0.x, n0.y, n0.z), nb = mk(n0.w, n1.x, n1.y),
+                                         nc = mk(n1.z, n1.w, n2.x);
+                                // weights (1 - cx - cy, cx, cy), as Sample.h:121 applies them
+                                lN = normalize(na * (1 - cx - cy) + nb * cx + nc * cy);
+                            }
+                            else
+                                lN = normalize(cross(p1 - p0, p2 - p0));
+                            V3 visRay = lpos - rayPos;
+                            const float dist = length(visRay);
+                            visRay = visRay / dist;
+                            const float cosTheta1 = dot(lN, visRay);
+                            if (lt.two_sided || cosTheta1 < 0)
+                            {
+                                const float cosTheta2 = dot(Ns, visRay);
+                                const float neePdf = lpdf * dist * dist / fabsf(cosTheta1);
+                                float uptPdf = 0.f;
+                                if (P.strategy != WL_STRATEGY_NEE)
+                                {
+                                    uptPdf = pdf_disney(mat, Ns, -lastRayDir, visRay);
+                                    if (uptPdf < 0 || __builtin_isnan(uptPdf))
+                                        uptPdf = 0.f;
+                                }
+                                V3 bv = eval_disney(mat, Ns, Ng, -lastRayDir, visRay, mk(1, 1, 1));
+                                bv = clamp3(bv, 0.f, 1e30f);
+                                contrib = throughput * bv * fabsf(cosTheta2) * ld3(lt.L) / neePdf *
+                                          (1 - uptPdf / (uptPdf + neePdf));
+                                shadow = true;
+                                shadow_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
+                                                     visRay.x, visRay.y, visRay.z,
+                                                     dist * (1 - P.epsilon) };
+                            }
+                        }
+                        // ---- Russian roulette (PathTracing.cu:130-138)
+                        if (depth > P.rr_depth)
+                        {
+                            if (rnd(seed) > P.rr_prob)
+                            {
+                                throughput = mk(0, 0, 0);
+                                done = true;
+                            }
+                            else
+                                throughput = throughput / P.rr_prob;
+                        }
+                    }
+                }
+            }
+            B.rad[path] = f4(radiance, 0.0f);
+            if (!done)
+            {
+                cont = true;
+                B.pos[path] = f4(rayPos, lastPdf);
+                B.dir[path] = f4(rayDir, __uint_as_float(seed));
+                B.thr[path] = f4(throughput, __uint_as_float(depth));
+                B.ltt[path] = f4(lastTraceTerm, 0.0f);
+                next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
+                                   rayDir.x, rayDir.y, rayDir.z, 1e30f };
+            }
+        }
+        // ---- compaction (whole wave participates)
+        const uint32_t slot = wave_append(&B.counters[parity ^ 1], cont);
+        if (cont)
+        {
+            qout[slot] = path;
+            rout[slot] = next_ray;
+        }
+        const uint32_t sslot = wave_append(&B.counters[2], shadow);
+        if (shadow)
+        {
+            B.spath[sslot] = path;
+            B.srays[sslot] = shadow_ray;
+            B.scontrib[sslot] = f4(contrib, 0.0f);
+        }
+    }
+}
+
+// ---- shadow: any hit; unoccluded rays add their NEE contribution ----------
+__global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
+{
+    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kPtBlock];
+    const uint32_t count = B.counters[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        B.stats[1] += count;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
+    {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < count;
+        TravRay r;
+        if (active)
+        {
+            const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
+            const float4 a = rp[0], b = rp[1];
+            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+        }
+        else
+            trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
+        uint32_t nn = 0, nt = 0;
+        TravResult h = trav_run<true, false, kPtBlock>(S.nodes, S.tris, r, active,
+                                                      stack + threadIdx.x, nn, nt);
+        if (active && h.prim == WL_INVALID)
+        {
+            const uint32_t path = B.spath[i];
+            const float4 c = B.scontrib[i];
+            float4 rr = B.rad[path];
+            rr.x = rr.x + c.x;
+            rr.y = rr.y + c.y;
+            rr.z = rr.z + c.z;
+            B.rad[path] = rr;
+        }
+    }
+}
+
+// ---- film -----------------------------------------------------------------
+// Kahan-compensated sum over the batch's samples of each pixel, in sample
+// order: film = [sum RGB, compensation RGB] per pixel.
+__global__ __launch_bounds__(kPtBlock) void k_accumulate(PtParams P, PtBuffers B, float *film)
+{
+    for (uint32_t px = blockIdx.x * kPtBlock + threadIdx.x; px < P.npix;
+         px += gridDim.x * kPtBlock)
+    {
+        float *f = film + 6 * ((size_t)P.pix0 + px);
+        float sum[3] = { f[0], f[1], f[2] }, c[3] = { f[3], f[4], f[5] };
+        for (uint32_t s = 0; s < P.nsamples; ++s)
+        {
+            const float4 r = B.rad[(size_t)s * P.npix + px];
+            const float v[3] = { r.x, r.y, r.z };
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                const float y = v[k] - c[k];
+                const float t = sum[k] + y;
+                c[k] = (t - sum[k]) - y;
+                sum[k] = t;
+            }
+        }
+        for (int k = 0; k < 3; ++k)
+        {
+            f[k] = sum[k];
+            f[3 + k] = c[k];
+        }
+    }
+}
+
+__global__ void k_store_samples(PtParams P, PtBuffers B, float *rgb)
+{
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < P.npix;
+         px += gridDim.x * blockDim.x)
+    {
+        const float4 r = B.rad[px];
+        const size_t o = 3 * ((size_t)P.pix0 + px);
+        rgb[o] = r.x;
+        rgb[o + 1] = r.y;
+        rgb[o + 2] = r.z;
+    }
+}
+
+// Reference progressive film (PathTracing.cu:140-158): running mean with
+// float factors and double temporaries in an RGBA float buffer, clamp,
+// ACES, x255, truncate to u8.
+__global__ void k_frame_update(PtParams P, PtBuffers B, uint32_t frame, float *acc,
+                               uint8_t *color)
+{
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < P.npix;
+         px += gridDim.x * blockDim.x)
+    {
+        const float4 r = B.rad[px];
+        const double tf[4] = { 0.0 + (double)(r.x / 1), 0.0 + (double)(r.y / 1),
+                               0.0 + (double)(r.z / 1), 1.0 };
+        float *a = acc + 4 * ((size_t)P.pix0 + px);
+        if (frame == 0)
+        {
+            for (int k = 0; k < 4; ++k)
+                a[k] = (float)tf[k];
+        }
+        else
+        {
+            const double f0 = (double)((float)frame / ((float)frame + 1.0f));
+            const double f1 = (double)(1.0f / ((float)frame + 1.0f));
+            for (int k = 0; k < 4; ++k)
+                a[k] = (float)((double)a[k] * f0 + tf[k] * f1);
+        }
+        uint8_t *c = color + 4 * ((size_t)P.pix0 + px);
+        for (int k = 0; k < 3; ++k)
+            c[k] = (uint8_t)(aces1(clampf(a[k], 0.f, 1e30f)) * 255.0f);
+        c[3] = (uint8_t)(1.0f * 255.0f);
+    }
+}
+
+__global__ void k_resolve(const float *film, uint32_t npix, uint32_t nsamples, float *rgb)
+{
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix;
+         px += gridDim.x * blockDim.x)
+    {
+        for (int k = 0; k < 3; ++k)
+            rgb[3 * (size_t)px + k] =
+                (film[6 * (size_t)px + k] - film[6 * (size_t)px + 3 + k]) / (float)nsamples;
+    }
+}
+
+// ---- host side: one batch ---------------------------------------------------
+hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
+                        int num_cus, hipStream_t st)
+{
+    const uint32_t npaths = P.npix * P.nsamples;
+    const uint32_t blocks_all = (npaths + kPtBlock - 1) / kPtBlock;
+    const uint32_t g_gen = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
+    const uint32_t g_trav = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 5);
+    const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
+    hipLaunchKernelGGL(k_raygen, dim3(g_gen), dim3(kPtBlock), 0, st, P, B);
+    for (uint32_t b = 0; b < P.max_depth; ++b)
+    {
+        const int parity = (int)(b & 1u);
+        hipLaunchKernelGGL(k_extend, dim3(g_trav), dim3(kPtBlock), 0, st, S, B, parity);
+        hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B, parity);
+        hipLaunchKernelGGL(k_shadow, dim3(g_trav), dim3(kPtBlock), 0, st, S, B);
+    }
+    return hipGetLastError();
+}
+
+hipError_t pt_accumulate(const PtParams &P, const PtBuffers &B, float *film, int num_cus,
+                         hipStream_t st)
+{
+    const uint32_t g = std::min<uint32_t>((P.npix + kPtBlock - 1) / kPtBlock, (uint32_t)num_cus * 8);
+    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kPtBlock), 0, st, P, B, film);
+    return hipGetLastError();
+}
+
+hipError_t pt_store_samples(const PtParams &P, const PtBuffers &B, float *rgb, hipStream_t st)
+{
+    const uint32_t g = std::min<uint32_t>((P.npix + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_store_samples, dim3(g), dim3(256), 0, st, P, B, rgb);
+    return hipGetLastError();
+}
+
+hipError_t pt_frame_update(const PtParams &P, const PtBuffers &B, uint32_t frame, float *acc,
+                           uint8_t *color, hipStream_t st)
+{
+    const uint32_t g = std::min<uint32_t>((P.npix + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_frame_update, dim3(g), dim3(256), 0, st, P, B, frame, acc, color);
+    return hipGetLastError();
+}
+
+hipError_t pt_resolve(const float *film, uint32_t npix, uint32_t nsamples, float *rgb,
+                      hipStream_t st)
+{
+    const uint32_t g = std::min<uint32_t>((npix + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_resolve, dim3(g), dim3(256), 0, st, film, npix, nsamples, rgb);
+    return hipGetLastError();
+}
+
+} // namespace wl

@@ -226,6 +226,9 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
         const wl_tri_input &in = inputs[i];
         a->input_offsets.push_back((uint32_t)off);
         a->input_tri_counts.push_back(in.num_triangles);
+        a->input_num_vertices.push_back(in.num_vertices);
+        a->h_indices.insert(a->h_indices.end(), in.indices,
+                            in.indices + 3 * (size_t)in.num_triangles);
         for (uint32_t t = 0; t < in.num_triangles; ++t)
         {
             for (int k = 0; k < 3; ++k)
@@ -282,6 +285,7 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     }
     a->num_nodes = (uint32_t)host.nodes.size();
     a->num_tris = total;
+    a->h_tris = std::move(tris);
     wl_accel_info &inf = a->info;
     inf.num_triangles = total;
     inf.num_inputs = num_inputs;
@@ -333,9 +337,12 @@ wl_status wl_accel_input_offsets(wl_accel a, uint32_t *offsets)
 }
 
 // ---- raw traversal --------------------------------------------------------
+// A NULL stream is HIP's null stream, as in every HIP API; the context's own
+// stream is available through wl_context_get_stream.
 static hipStream_t pick_stream(wl_context ctx, void *stream)
 {
-    return stream ? (hipStream_t)stream : ctx->stream;
+    (void)ctx;
+    return (hipStream_t)stream;
 }
 
 wl_status wl_trace_closest(wl_accel a, const wl_ray *rays, uint32_t n,

@@ -26,46 +26,31 @@ struct wl_accel_s
     uint64_t num_tris = 0;
     std::vector<uint32_t> input_offsets;
     std::vector<uint32_t> input_tri_counts;
+    std::vector<uint32_t> input_num_vertices;
+    std::vector<float> h_tris;        // 9 floats per global primitive
+    std::vector<uint32_t> h_indices;  // 3 input-local vertex ids per primitive
     wl_accel_info info{};
 };
 
-// Device-side scene for the path tracer (structure of arrays).
-struct WlDeviceMesh
-{
-    uint32_t vertex_offset;   // into d_vertices / d_normals
-    uint32_t tri_offset;      // global primitive id of local triangle 0
-    uint32_t tri_count;
-    uint32_t has_normal;
-    uint32_t light;           // area light id or WL_INVALID_INDEX
-    uint32_t material;        // == mesh index
-    uint32_t pad[2];
-};
-
-struct WlDeviceLight
-{
-    uint32_t mesh;
-    uint32_t tri_count;
-    uint32_t two_sided;
-    uint32_t has_normal;
-    float L[3];
-    uint32_t pad;
-};
-
+// Path-tracer scene: device copies of the shading data plus the batch
+// buffers, grown on demand and kept for later calls.
 struct wl_scene_s
 {
     wl_context ctx = nullptr;
     wl_accel accel = nullptr;
     uint32_t num_meshes = 0, num_lights = 0;
-    float *d_vertices = nullptr;       // float3, mesh-local order
-    float *d_normals = nullptr;        // float3
-    uint32_t *d_indices = nullptr;     // uint3 per global triangle (mesh-local)
-    uint32_t *d_prim_mesh = nullptr;   // global triangle -> mesh
-    WlDeviceMesh *d_meshes = nullptr;
+    float *d_prims = nullptr;                 // 12 floats per primitive
+    float *d_normals = nullptr;               // 12 floats per primitive
+    void *d_meshes = nullptr;                 // wl::PtMesh[num_meshes]
     wl_disney_material *d_materials = nullptr;
-    WlDeviceLight *d_lights = nullptr;
+    void *d_lights = nullptr;                 // wl::PtLight[num_lights]
     wl_pinhole_camera camera{};
     float bg[3] = { 0, 0, 0 };
-    std::vector<WlDeviceMesh> h_meshes;
+    // batch buffers
+    uint32_t capacity = 0;
+    void *d_batch = nullptr;                  // one allocation, carved up
+    uint32_t *d_counters = nullptr;
+    unsigned long long *d_stats = nullptr;
 };
 
 namespace wl

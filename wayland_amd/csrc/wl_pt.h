@@ -1,0 +1,88 @@
+// wl_pt.h — device data of the wavefront path tracer (pathtrace.hip) and the
+// host functions that enqueue it (used by wl_pt_api.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wayland_hip.h"
+#include "bvh8.h"
+
+namespace wl
+{
+
+struct PtMesh
+{
+    uint32_t has_normal;
+    uint32_t light;     // area light id or WL_INVALID_INDEX
+    uint32_t material;  // index into materials
+    uint32_t tri_offset;
+};
+
+struct PtLight
+{
+    uint32_t tri_offset; // global primitive id of the light mesh's triangle 0
+    uint32_t tri_count;
+    uint32_t two_sided;
+    uint32_t has_normal;
+    float L[3];
+    uint32_t mesh;
+};
+
+// Scene as the kernels read it.
+//   prims:   per global primitive, 3 x float4 = v0.xyz v1.x | v1.yz v2.xy |
+//            v2.z, mesh, local index, 0   (48 bytes, primitive order)
+//   normals: per global primitive, 3 x float4 = n0.xyz n1.x | n1.yz n2.xy |
+//            n2.z, 0, 0, 0 (only read for meshes with normals)
+struct PtScene
+{
+    const WlBvh8Node *nodes;
+    const WlTriRecord *tris;
+    const float *prims;
+    const float *normals;
+    const PtMesh *meshes;
+    const wl_disney_material *materials;
+    const PtLight *lights;
+};
+
+// One batch = pixels [pix0, pix0 + npix) x samples [frame0, frame0 + nsamples)
+struct PtParams
+{
+    uint32_t width, height;
+    uint32_t pix0, npix, nsamples, frame0;
+    uint32_t max_depth, rr_depth;
+    float rr_prob, epsilon;
+    uint32_t strategy, num_lights;
+    float bg[3];
+    wl_pinhole_camera cam;
+};
+
+// Path state (structure of float4 arrays, one entry per path) and queues.
+struct PtBuffers
+{
+    float4 *pos;      // ray origin, w = lastPdf
+    float4 *dir;      // ray direction, w = seed bits
+    float4 *thr;      // throughput, w = depth bits
+    float4 *ltt;      // lastTraceTerm
+    float4 *rad;      // radiance
+    wl_ray *rays0, *rays1;
+    uint32_t *qpath0, *qpath1;
+    wl_hit *hits;
+    wl_ray *srays;
+    uint32_t *spath;
+    float4 *scontrib;
+    uint32_t *counters;       // [0], [1] active queues, [2] shadow queue
+    unsigned long long *stats; // [0] extension rays, [1] shadow rays
+};
+
+hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
+                        int num_cus, hipStream_t st);
+hipError_t pt_accumulate(const PtParams &P, const PtBuffers &B, float *film,
+                         int num_cus, hipStream_t st);
+hipError_t pt_store_samples(const PtParams &P, const PtBuffers &B, float *rgb,
+                            hipStream_t st);
+hipError_t pt_frame_update(const PtParams &P, const PtBuffers &B, uint32_t frame,
+                           float *acc, uint8_t *color, hipStream_t st);
+hipError_t pt_resolve(const float *film, uint32_t npix, uint32_t nsamples,
+                      float *rgb, hipStream_t st);
+
+} // namespace wl

@@ -1,0 +1,396 @@
+// wl_pt_api.cpp — C ABI of the path tracer: scene set-up (the device buffer
+// layout of src/Frontend/Core/DeviceManager.cpp:77-181 + the hit records of
+// PathTracingProgramManager.cpp:57-132, pre-gathered per primitive) and the
+// render entry points.
+#include "wl_internal.h"
+#include "wl_pt.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+
+using wl::set_error;
+
+namespace
+{
+
+// Carves the batch buffers for `cap` paths out of one allocation.
+wl::PtBuffers carve(wl_scene s)
+{
+    wl::PtBuffers B{};
+    const size_t cap = s->capacity;
+    char *p = (char *)s->d_batch;
+    auto take = [&](size_t bytes) {
+        char *q = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return (void *)q;
+    };
+    B.pos = (float4 *)take(16 * cap);
+    B.dir = (float4 *)take(16 * cap);
+    B.thr = (float4 *)take(16 * cap);
+    B.ltt = (float4 *)take(16 * cap);
+    B.rad = (float4 *)take(16 * cap);
+    B.rays0 = (wl_ray *)take(32 * cap);
+    B.rays1 = (wl_ray *)take(32 * cap);
+    B.qpath0 = (uint32_t *)take(4 * cap);
+    B.qpath1 = (uint32_t *)take(4 * cap);
+    B.hits = (wl_hit *)take(16 * cap);
+    B.srays = (wl_ray *)take(32 * cap);
+    B.spath = (uint32_t *)take(4 * cap);
+    B.scontrib = (float4 *)take(16 * cap);
+    B.counters = s->d_counters;
+    B.stats = s->d_stats;
+    return B;
+}
+
+size_t batch_bytes(size_t cap)
+{
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 3 + r(16 * cap) * 2;
+}
+
+wl_status ensure_capacity(wl_scene s, uint32_t paths)
+{
+    if (paths <= s->capacity)
+        return WL_SUCCESS;
+    if (s->d_batch)
+    {
+        WL_HIP_CHECK(hipDeviceSynchronize());
+        WL_HIP_CHECK(hipFree(s->d_batch));
+        s->d_batch = nullptr;
+        s->capacity = 0;
+    }
+    WL_HIP_CHECK(hipMalloc(&s->d_batch, batch_bytes(paths)));
+    s->capacity = paths;
+    return WL_SUCCESS;
+}
+
+wl::PtScene scene_view(wl_scene s)
+{
+    wl::PtScene S;
+    S.nodes = s->accel->d_nodes;
+    S.tris = s->accel->d_tris;
+    S.prims = s->d_prims;
+    S.normals = s->d_normals;
+    S.meshes = (const wl::PtMesh *)s->d_meshes;
+    S.materials = s->d_materials;
+    S.lights = (const wl::PtLight *)s->d_lights;
+    return S;
+}
+
+wl_status check_config(wl_scene s, const wl_pt_config *c)
+{
+    if (!s || !c)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (c->width == 0 || c->height == 0)
+        return set_error(WL_ERROR_INVALID_VALUE, "empty image");
+    if ((uint64_t)c->width * c->height > 0x7FFFFFFFull)
+        return set_error(WL_ERROR_LIMIT_EXCEEDED, "image too large");
+    if (c->max_depth == 0 || c->max_depth > 1024)
+        return set_error(WL_ERROR_INVALID_VALUE, "max_depth must be in [1, 1024]");
+    if (!(c->rr_prob > 0.0f && c->rr_prob <= 1.0f))
+        return set_error(WL_ERROR_INVALID_VALUE, "rr_prob must be in (0, 1]");
+    if (c->strategy > WL_STRATEGY_MIS)
+        return set_error(WL_ERROR_INVALID_VALUE, "unknown strategy");
+    return WL_SUCCESS;
+}
+
+wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
+{
+    wl::PtParams P{};
+    P.width = c->width;
+    P.height = c->height;
+    P.max_depth = c->max_depth;
+    P.rr_depth = c->rr_depth;
+    P.rr_prob = c->rr_prob;
+    P.epsilon = c->epsilon;
+    P.strategy = c->strategy;
+    P.num_lights = s->num_lights;
+    std::memcpy(P.bg, s->bg, sizeof P.bg);
+    P.cam = s->camera;
+    return P;
+}
+
+constexpr uint32_t kDefaultBatchPaths = 1u << 23;
+
+} // namespace
+
+extern "C" {
+
+wl_status wl_scene_create(wl_context ctx, wl_accel accel,
+                          const wl_mesh_shading *meshes, uint32_t num_meshes,
+                          const wl_area_light *lights, uint32_t num_lights,
+                          const wl_pinhole_camera *camera, const float bg[3],
+                          wl_scene *out)
+{
+    if (!ctx || !accel || !out || (num_meshes && !meshes) || (num_lights && !lights))
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    *out = nullptr;
+    if (num_meshes != accel->input_offsets.size())
+        return set_error(WL_ERROR_INVALID_VALUE,
+                         "one wl_mesh_shading per build input is required");
+    for (uint32_t l = 0; l < num_lights; ++l)
+    {
+        if (lights[l].mesh >= num_meshes)
+            return set_error(WL_ERROR_INVALID_VALUE, "light mesh out of range");
+        if (accel->input_tri_counts[lights[l].mesh] == 0)
+            return set_error(WL_ERROR_INVALID_VALUE, "light mesh has no triangles");
+    }
+    for (uint32_t m = 0; m < num_meshes; ++m)
+        if (meshes[m].area_light != WL_INVALID_INDEX && meshes[m].area_light >= num_lights)
+            return set_error(WL_ERROR_INVALID_VALUE, "mesh area_light out of range");
+    const size_t T = accel->num_tris;
+    std::vector<float> prims(12 * std::max<size_t>(1, T), 0.0f);
+    std::vector<float> normals;
+    bool any_normals = false;
+    for (uint32_t m = 0; m < num_meshes; ++m)
+        any_normals |= meshes[m].normals != nullptr;
+    if (any_normals)
+        normals.assign(12 * std::max<size_t>(1, T), 0.0f);
+    std::vector<wl::PtMesh> hm(num_meshes);
+    for (uint32_t m = 0; m < num_meshes; ++m)
+    {
+        const uint32_t off = accel->input_offsets[m], cnt = accel->input_tri_counts[m];
+        hm[m].has_normal = meshes[m].normals ? 1u : 0u;
+        hm[m].light = meshes[m].area_light;
+        hm[m].material = m;
+        hm[m].tri_offset = off;
+        for (uint32_t t = 0; t < cnt; ++t)
+        {
+            const size_t g = (size_t)off + t;
+            float *r = &prims[12 * g];
+            std::memcpy(r, &accel->h_tris[9 * g], 36);
+            std::memcpy(&r[9], &m, 4);
+            std::memcpy(&r[10], &t, 4);
+            if (meshes[m].normals)
+            {
+                float *n = &normals[12 * g];
+                for (int k = 0; k < 3; ++k)
+                {
+                    uint32_t v = accel->h_indices[3 * g + k];
+                    std::memcpy(n + 3 * k, meshes[m].normals + 3 * (size_t)v, 12);
+                }
+            }
+        }
+    }
+    std::vector<wl::PtLight> hl(std::max<uint32_t>(1, num_lights));
+    for (uint32_t l = 0; l < num_lights; ++l)
+    {
+        const uint32_t m = lights[l].mesh;
+        hl[l].tri_offset = accel->input_offsets[m];
+        hl[l].tri_count = accel->input_tri_counts[m];
+        hl[l].two_sided = lights[l].two_sided ? 1u : 0u;
+        hl[l].has_normal = meshes[m].normals ? 1u : 0u;
+        std::memcpy(hl[l].L, lights[l].L, 12);
+        hl[l].mesh = m;
+    }
+    std::vector<wl_disney_material> mats(std::max<uint32_t>(1, num_meshes));
+    for (uint32_t m = 0; m < num_meshes; ++m)
+        mats[m] = meshes[m].material;
+
+    wl_scene s = new (std::nothrow) wl_scene_s;
+    if (!s)
+        return set_error(WL_ERROR_OUT_OF_MEMORY, "scene allocation");
+    s->ctx = ctx;
+    s->accel = accel;
+    s->num_meshes = num_meshes;
+    s->num_lights = num_lights;
+    if (camera)
+        s->camera = *camera;
+    if (bg)
+        std::memcpy(s->bg, bg, sizeof s->bg);
+    hipError_t e = hipSetDevice(ctx->device);
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess)
+            return;
+        e = hipMalloc(dst, bytes);
+        if (e == hipSuccess)
+            e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up((void **)&s->d_prims, prims.data(), prims.size() * 4);
+    if (any_normals)
+        up((void **)&s->d_normals, normals.data(), normals.size() * 4);
+    up(&s->d_meshes, hm.data(), std::max<size_t>(1, hm.size()) * sizeof(wl::PtMesh));
+    up((void **)&s->d_materials, mats.data(), mats.size() * sizeof(wl_disney_material));
+    up(&s->d_lights, hl.data(), hl.size() * sizeof(wl::PtLight));
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_counters, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_stats, 64);
+    if (e != hipSuccess)
+    {
+        wl_scene_destroy(s);
+        return wl::hip_error(e, "scene upload");
+    }
+    *out = s;
+    return WL_SUCCESS;
+}
+
+wl_status wl_scene_destroy(wl_scene s)
+{
+    if (!s)
+        return set_error(WL_ERROR_INVALID_VALUE, "scene is NULL");
+    hipSetDevice(s->ctx->device);
+    hipDeviceSynchronize();
+    hipFree(s->d_prims);
+    hipFree(s->d_normals);
+    hipFree(s->d_meshes);
+    hipFree(s->d_materials);
+    hipFree(s->d_lights);
+    hipFree(s->d_batch);
+    hipFree(s->d_counters);
+    hipFree(s->d_stats);
+    delete s;
+    return WL_SUCCESS;
+}
+
+wl_status wl_scene_set_camera(wl_scene s, const wl_pinhole_camera *camera)
+{
+    if (!s || !camera)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    s->camera = *camera;
+    return WL_SUCCESS;
+}
+
+wl_status wl_pt_render(wl_scene s, const wl_pt_config *c, float *film,
+                       wl_pt_stats *stats, void *stream)
+{
+    wl_status st = check_config(s, c);
+    if (st != WL_SUCCESS)
+        return st;
+    if (!film)
+        return set_error(WL_ERROR_INVALID_VALUE, "film is NULL");
+    if (c->spp == 0)
+        return WL_SUCCESS;
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    auto t0 = std::chrono::steady_clock::now();
+    const uint32_t npix_total = c->width * c->height;
+    const uint32_t max_paths = c->max_batch_paths ? c->max_batch_paths : kDefaultBatchPaths;
+    // batch = a pixel range x a sample range
+    uint32_t tile = std::min(npix_total, max_paths);
+    uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(c->spp, max_paths / tile));
+    st = ensure_capacity(s, tile * ns);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtBuffers B = carve(s);
+    wl::PtScene S = scene_view(s);
+    if (stats)
+        WL_HIP_CHECK(hipMemsetAsync(s->d_stats, 0, 16, hs));
+    for (uint32_t s0 = 0; s0 < c->spp; s0 += ns)
+    {
+        const uint32_t nss = std::min(ns, c->spp - s0);
+        for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
+        {
+            wl::PtParams P = params_for(s, c);
+            P.pix0 = p0;
+            P.npix = std::min(tile, npix_total - p0);
+            P.nsamples = nss;
+            P.frame0 = c->frame_offset + s0;
+            WL_HIP_CHECK(wl::pt_run_batch(P, S, B, s->ctx->num_cus, hs));
+            WL_HIP_CHECK(wl::pt_accumulate(P, B, film, s->ctx->num_cus, hs));
+        }
+    }
+    if (stats)
+    {
+        unsigned long long h[2] = { 0, 0 };
+        WL_HIP_CHECK(hipMemcpyAsync(h, s->d_stats, 16, hipMemcpyDeviceToHost, hs));
+        WL_HIP_CHECK(hipStreamSynchronize(hs));
+        auto t1 = std::chrono::steady_clock::now();
+        stats->paths = (uint64_t)npix_total * c->spp;
+        stats->extension_rays = h[0];
+        stats->shadow_rays = h[1];
+        stats->max_bounce = c->max_depth;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return WL_SUCCESS;
+}
+
+static wl_status one_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
+                           wl::PtBuffers &B, hipStream_t hs, uint32_t p0, uint32_t np)
+{
+    wl::PtParams P = params_for(s, c);
+    P.pix0 = p0;
+    P.npix = np;
+    P.nsamples = 1;
+    P.frame0 = frame;
+    WL_HIP_CHECK(wl::pt_run_batch(P, scene_view(s), B, s->ctx->num_cus, hs));
+    return WL_SUCCESS;
+}
+
+wl_status wl_pt_render_samples(wl_scene s, const wl_pt_config *c, uint32_t frame,
+                               float *rgb, void *stream)
+{
+    wl_status st = check_config(s, c);
+    if (st != WL_SUCCESS)
+        return st;
+    if (!rgb)
+        return set_error(WL_ERROR_INVALID_VALUE, "rgb is NULL");
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    const uint32_t npix_total = c->width * c->height;
+    const uint32_t max_paths = c->max_batch_paths ? c->max_batch_paths : kDefaultBatchPaths;
+    const uint32_t tile = std::min(npix_total, max_paths);
+    st = ensure_capacity(s, tile);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtBuffers B = carve(s);
+    for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
+    {
+        const uint32_t np = std::min(tile, npix_total - p0);
+        st = one_frame(s, c, frame, B, hs, p0, np);
+        if (st != WL_SUCCESS)
+            return st;
+        wl::PtParams P = params_for(s, c);
+        P.pix0 = p0;
+        P.npix = np;
+        P.nsamples = 1;
+        WL_HIP_CHECK(wl::pt_store_samples(P, B, rgb, hs));
+    }
+    return WL_SUCCESS;
+}
+
+wl_status wl_pt_launch_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
+                             float *radiance_rgba, uint8_t *color_rgba, void *stream)
+{
+    wl_status st = check_config(s, c);
+    if (st != WL_SUCCESS)
+        return st;
+    if (!radiance_rgba || !color_rgba)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL buffer");
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    const uint32_t npix_total = c->width * c->height;
+    const uint32_t max_paths = c->max_batch_paths ? c->max_batch_paths : kDefaultBatchPaths;
+    const uint32_t tile = std::min(npix_total, max_paths);
+    st = ensure_capacity(s, tile);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtBuffers B = carve(s);
+    for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
+    {
+        const uint32_t np = std::min(tile, npix_total - p0);
+        st = one_frame(s, c, frame, B, hs, p0, np);
+        if (st != WL_SUCCESS)
+            return st;
+        wl::PtParams P = params_for(s, c);
+        P.pix0 = p0;
+        P.npix = np;
+        P.nsamples = 1;
+        WL_HIP_CHECK(wl::pt_frame_update(P, B, frame, radiance_rgba, color_rgba, hs));
+    }
+    return WL_SUCCESS;
+}
+
+wl_status wl_film_resolve(wl_context ctx, const float *film, uint32_t npix,
+                          uint32_t nsamples, float *rgb, void *stream)
+{
+    if (!ctx || !film || !rgb || nsamples == 0)
+        return set_error(WL_ERROR_INVALID_VALUE, "bad argument");
+    WL_HIP_CHECK(hipSetDevice(ctx->device));
+    WL_HIP_CHECK(wl::pt_resolve(film, npix, nsamples, rgb, (hipStream_t)stream));
+    return WL_SUCCESS;
+}
+
+} // extern "C"
