@@ -245,6 +245,144 @@ wl_status wl_pt_launch_frame(wl_scene scene, const wl_pt_config *cfg,
 wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
                           uint32_t nsamples, float *d_rgb, void *stream);
 
+/* ---- programs, pipeline, SBT, launch: the OptiX program interface -------
+ *
+ * Program discovery by name, as Module::IdentifyPrograms does it
+ * (src/Backend/Optix/Core/Module-AutoConfig.cpp:76-178): every
+ *   extern "C" __global__ void __<kind>__<name>()
+ * in the source text is a program; raygen and miss programs become groups in
+ * file order; intersection/any-hit/closest-hit programs whose names end with
+ * the same suffix merge into one hit group; hit groups follow all raygen and
+ * miss groups, in order of first appearance.  The device code itself is not
+ * compiled: a pipeline maps the discovered program set to a built-in gfx950
+ * integrator (the PathTracing program of src/Frontend/Programs/PathTracing,
+ * or the Lambertian program of examples/PathTracer).                       */
+enum
+{
+    WL_PROGRAM_RAYGEN = 0,
+    WL_PROGRAM_MISS = 1,
+    WL_PROGRAM_HITGROUP = 2
+};
+
+typedef struct wl_program_group
+{
+    uint32_t kind;
+    /* raygen/miss: entry[0]; hit group: entry[0] intersection,
+     * entry[1] any-hit, entry[2] closest-hit ("" when absent) */
+    char entry[3][64];
+} wl_program_group;
+
+wl_status wl_identify_programs(const char *source, size_t length,
+                               wl_program_group *groups, uint32_t capacity,
+                               uint32_t *count);
+
+typedef struct wl_pipeline_s *wl_pipeline;
+typedef struct wl_sbt_s *wl_sbt;
+
+enum
+{
+    WL_BUILTIN_PATHTRACING = 1,      /* PathTracing.cu program set         */
+    WL_BUILTIN_EXAMPLE_PATHTRACER = 2 /* examples/PathTracer/PathTracer.cu */
+};
+
+/* Pipeline (Pipeline.cpp:93-136): WL_ERROR_UNKNOWN_PROGRAM when the program
+ * set is not one of the built-in integrators.  *builtin gets its id. */
+wl_status wl_pipeline_create(wl_context ctx, const wl_program_group *groups,
+                             uint32_t num_groups, uint32_t max_trace_depth,
+                             wl_pipeline *out, uint32_t *builtin);
+wl_status wl_pipeline_destroy(wl_pipeline p);
+
+/* SBT records keep the reference's layout: a 32-byte header followed by the
+ * program's data, 16-byte aligned (SBTData.h:8-25).  wl_sbt_pack_header
+ * writes the header for program group `group_index` (optixSbtRecordPackHeader).
+ * Hit record of build input i and ray type r = hit_records[r + ray_type_count*i]
+ * (the addressing of GetSBTHitRecordBuffer, Traversable.h:95-102). */
+#define WL_SBT_RECORD_HEADER_SIZE 32
+#define WL_SBT_RECORD_ALIGNMENT 16
+wl_status wl_sbt_pack_header(wl_pipeline p, uint32_t group_index, void *record);
+wl_status wl_sbt_create(wl_pipeline p, const void *raygen_record, size_t raygen_size,
+                        const void *miss_records, uint32_t miss_count,
+                        size_t miss_stride, const void *hit_records,
+                        uint32_t hit_count, size_t hit_stride, wl_sbt *out);
+wl_status wl_sbt_destroy(wl_sbt sbt);
+
+/* The value a LaunchParams `traversable` field holds for this accel. */
+wl_status wl_accel_traversable(wl_accel accel, uint64_t *handle);
+
+/* optixLaunch (Launcher.cpp:12-20): params is host memory laid out as the
+ * program's LaunchParams (copied at launch, like Launcher's device buffer):
+ *   PathTracing:      wl_pathtracing_params (PathTracingLaunchParams.h:44-57)
+ *   examples program: wl_example_params      (examples/PathTracer/Param.h:11-18) */
+wl_status wl_launch(wl_pipeline p, void *stream, const void *params,
+                    size_t params_size, wl_sbt sbt, uint32_t width,
+                    uint32_t height, uint32_t depth);
+
+/* Reference device structs the two built-in programs read (same layouts). */
+typedef struct wl_pathtracing_params
+{
+    uint32_t frameID;
+    int32_t fbSize[2];
+    uint32_t pad0_;
+    float *radianceBuffer;         /* vec4 per pixel                        */
+    uint8_t *colorBuffer;          /* u8vec4 per pixel                      */
+    wl_pinhole_camera camera;
+    uint32_t areaLightCount;
+    uint32_t pad1_;
+    const void *areaLights;        /* wl_device_area_light[areaLightCount]  */
+    uint64_t traversable;
+} wl_pathtracing_params;
+
+typedef struct wl_device_area_light /* Device/Light.h:9-17 */
+{
+    uint8_t twoSided, hasNormal, pad0_[2];
+    uint32_t triangleNum;
+    float L[3];
+    uint32_t pad1_;
+    const float *vertices;
+    const float *normals;
+    const int32_t *indices;
+} wl_device_area_light;
+
+typedef struct wl_pathtracing_hit_data /* PathTracingLaunchParams.h:64-78 */
+{
+    uint8_t hasTexture, hasNormal, pad0_[6];
+    uint64_t texture;
+    uint32_t meshID, areaLightID, materialID;
+    float disneyMat[15];           /* DisneyMaterial, 60 bytes               */
+    const int32_t *indices;
+    const float *vertices;
+    const float *normals;
+    const float *texcoords;
+} wl_pathtracing_hit_data;
+
+typedef struct wl_example_camera /* DeviceUtils/Camera.h:7-14 */
+{
+    float horizontalRatio, verticalRatio;
+    float position[3], gaze[3], up[3];
+} wl_example_camera;
+
+typedef struct wl_example_params /* examples/PathTracer/Param.h:11-18 */
+{
+    uint32_t sampleNum;
+    uint32_t maxDepth;
+    wl_example_camera camera;
+    uint32_t pad_;
+    float *colorBuffer;            /* vec4 per pixel                        */
+    uint64_t traversable;
+} wl_example_params;
+
+typedef struct wl_example_hit_data /* examples/PathTracer/Param.h:20-28 */
+{
+    float color[3];
+    float emission[3];
+    const int32_t *indices;        /* device, ivec3 per triangle            */
+    const float *normals;          /* device, vec3 per vertex               */
+} wl_example_hit_data;
+
+/* examples/PathTracer/Conversion.cu:6-19 FromFloatToChar: v > 1 -> 1, then
+ * (u8)(v * 255), on the device. */
+wl_status wl_float_to_u8(const float *d_src, size_t n, uint8_t *d_dst, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

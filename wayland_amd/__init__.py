@@ -166,6 +166,101 @@ def pt_config(width, height, spp=1, frame_offset=0, max_depth=25, rr_depth=5,
                          rr_prob, epsilon, strategy, flags, max_batch_paths, 0)
 
 
+def identify_programs(source: str):
+    """Module::IdentifyPrograms on source text -> list of (kind, entries)."""
+    data = source.encode()
+    cap = 64
+    arr = (_lib.ProgramGroup * cap)()
+    n = C.c_uint32()
+    _lib.call("wl_identify_programs", data, len(data), arr, cap, C.byref(n))
+    return [(arr[i].kind, tuple(arr[i].entry[k].value.decode() for k in range(3)))
+            for i in range(n.value)], arr, n.value
+
+
+class Pipeline:
+    """Program groups -> a built-in gfx950 integrator (reference Pipeline)."""
+
+    def __init__(self, ctx: Context, source: str, max_trace_depth: int = 1):
+        self.ctx = ctx
+        self.groups, arr, n = identify_programs(source)
+        h = C.c_void_p()
+        b = C.c_uint32()
+        _lib.call("wl_pipeline_create", ctx.h, arr, n, max_trace_depth, C.byref(h), C.byref(b))
+        self.h, self.builtin = h, b.value
+
+    def group_index(self, entry: str) -> int:
+        for i, (_, names) in enumerate(self.groups):
+            if entry in names:
+                return i
+        raise KeyError(entry)
+
+    def record(self, group_index: int, data=None) -> bytes:
+        """One SBT record: packed header + data, padded to 16 bytes."""
+        size = 0 if data is None else C.sizeof(data)
+        buf = (C.c_uint8 * _lib.record_stride(size))()
+        _lib.call("wl_sbt_pack_header", self.h, group_index, buf)
+        if data is not None:
+            C.memmove(C.addressof(buf) + _lib.SBT_HEADER, C.addressof(data), size)
+        return bytes(buf)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_pipeline_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SBT:
+    """Shader binding table (reference ShaderBindingTable)."""
+
+    def __init__(self, pipeline: Pipeline, raygen: bytes, miss: list, hit: list):
+        self.pipeline = pipeline
+        ms = len(miss[0]) if miss else 0
+        hs = len(hit[0]) if hit else 0
+        mb = b"".join(miss)
+        hb = b"".join(hit)
+        h = C.c_void_p()
+        _lib.call("wl_sbt_create", pipeline.h, raygen, len(raygen), mb or None, len(miss), ms,
+                  hb or None, len(hit), hs, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_sbt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def traversable(accel: Accel) -> int:
+    v = C.c_uint64()
+    _lib.call("wl_accel_traversable", accel.h, C.byref(v))
+    return v.value
+
+
+def launch(pipeline: Pipeline, params, sbt: SBT, width: int, height: int, depth: int = 1):
+    """optixLaunch: params is a ctypes struct in the program's LaunchParams layout."""
+    _lib.call("wl_launch", pipeline.h, _stream_ptr(), C.byref(params), C.sizeof(params),
+              sbt.h, width, height, depth)
+
+
+def float_to_u8(src):
+    torch = _torch()
+    dst = torch.empty(src.numel(), dtype=torch.uint8, device=src.device)
+    _lib.call("wl_float_to_u8", C.c_void_p(src.data_ptr()), src.numel(),
+              C.c_void_p(dst.data_ptr()), _stream_ptr(src.device))
+    return dst
+
+
 class PathTracer:
     """The PathTracing program on a scene (reference:
     src/Frontend/Programs/PathTracing/*)."""

@@ -66,6 +66,54 @@ class PTStats(C.Structure):
                 ("pad_", C.c_uint32), ("seconds", C.c_double)]
 
 
+class ProgramGroup(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("entry", (C.c_char * 64) * 3)]
+
+
+class ExampleCamera(C.Structure):
+    _fields_ = [("horizontalRatio", C.c_float), ("verticalRatio", C.c_float),
+                ("position", C.c_float * 3), ("gaze", C.c_float * 3), ("up", C.c_float * 3)]
+
+
+class ExampleParams(C.Structure):
+    _fields_ = [("sampleNum", C.c_uint32), ("maxDepth", C.c_uint32),
+                ("camera", ExampleCamera), ("pad_", C.c_uint32),
+                ("colorBuffer", C.c_void_p), ("traversable", C.c_uint64)]
+
+
+class ExampleHitData(C.Structure):
+    _fields_ = [("color", C.c_float * 3), ("emission", C.c_float * 3),
+                ("indices", C.c_void_p), ("normals", C.c_void_p)]
+
+
+class PathTracingParams(C.Structure):
+    _fields_ = [("frameID", C.c_uint32), ("fbSize", C.c_int32 * 2), ("pad0_", C.c_uint32),
+                ("radianceBuffer", C.c_void_p), ("colorBuffer", C.c_void_p),
+                ("camera", PinholeCamera), ("areaLightCount", C.c_uint32),
+                ("pad1_", C.c_uint32), ("areaLights", C.c_void_p), ("traversable", C.c_uint64)]
+
+
+class DeviceAreaLight(C.Structure):
+    _fields_ = [("twoSided", C.c_uint8), ("hasNormal", C.c_uint8), ("pad0_", C.c_uint8 * 2),
+                ("triangleNum", C.c_uint32), ("L", C.c_float * 3), ("pad1_", C.c_uint32),
+                ("vertices", C.c_void_p), ("normals", C.c_void_p), ("indices", C.c_void_p)]
+
+
+class PathTracingHitData(C.Structure):
+    _fields_ = [("hasTexture", C.c_uint8), ("hasNormal", C.c_uint8), ("pad0_", C.c_uint8 * 6),
+                ("texture", C.c_uint64), ("meshID", C.c_uint32), ("areaLightID", C.c_uint32),
+                ("materialID", C.c_uint32), ("disneyMat", C.c_float * 15),
+                ("indices", C.c_void_p), ("vertices", C.c_void_p), ("normals", C.c_void_p),
+                ("texcoords", C.c_void_p)]
+
+
+SBT_HEADER = 32
+
+
+def record_stride(data_size):
+    return (SBT_HEADER + data_size + 15) // 16 * 16
+
+
 # name -> (restype, argtypes); every function declared in include/wayland_hip.h
 _VP = C.c_void_p
 _U32 = C.c_uint32
@@ -95,6 +143,16 @@ SIGNATURES = {
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
     "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),
     "wl_film_resolve": (C.c_int, [_VP, _VP, _U32, _U32, _VP, _VP]),
+    "wl_identify_programs": (C.c_int, [C.c_char_p, C.c_size_t, _VP, _U32, C.POINTER(_U32)]),
+    "wl_pipeline_create": (C.c_int, [_VP, _VP, _U32, _U32, C.POINTER(_VP), C.POINTER(_U32)]),
+    "wl_pipeline_destroy": (C.c_int, [_VP]),
+    "wl_sbt_pack_header": (C.c_int, [_VP, _U32, _VP]),
+    "wl_sbt_create": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _U32, C.c_size_t, _VP, _U32,
+                                C.c_size_t, C.POINTER(_VP)]),
+    "wl_sbt_destroy": (C.c_int, [_VP]),
+    "wl_accel_traversable": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    "wl_launch": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP, _U32, _U32, _U32]),
+    "wl_float_to_u8": (C.c_int, [_VP, C.c_size_t, _VP, _VP]),
 }
 
 _lib = None

@@ -26,8 +26,10 @@ SOURCES = [
     "bvh_builder.cpp",
     "wl_api.cpp",
     "wl_pt_api.cpp",
+    "wl_program.cpp",
     "traverse.hip",
     "pathtrace.hip",
+    "example_pt.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

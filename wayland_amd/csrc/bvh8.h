@@ -30,7 +30,8 @@
 // directions, Ylitie, Karras, Laine, HPG 2017).
 //
 // Triangle record: 48 bytes, world-space vertices gathered once at build time
-// plus the global primitive id (mesh triangle offset + local index).
+// plus the global primitive id (mesh triangle offset + local index), the
+// build input it came from and its index within that input.
 #pragma once
 #include <stdint.h>
 
@@ -54,7 +55,8 @@ struct alignas(16) WlTriRecord
     float v1[3];
     float v2[3];
     uint32_t prim;
-    uint32_t pad[2];
+    uint32_t input;
+    uint32_t local;
 };
 
 static_assert(sizeof(WlBvh8Node) == 128, "BVH8 node must be 128 bytes");

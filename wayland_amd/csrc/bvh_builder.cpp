@@ -404,7 +404,8 @@ bool build_bvh8(const float *tris, uint32_t ntris, int nthreads, Bvh8Host *out,
                     WlTriRecord r;
                     std::memcpy(r.v0, tris + 9 * (size_t)p, 36);
                     r.prim = p;
-                    r.pad[0] = r.pad[1] = 0;
+                    r.input = 0;
+                    r.local = p;
                     out->tris.push_back(r);
                 }
                 tri_off += k.count;

@@ -160,11 +160,12 @@ struct TravResult
     float t;
     uint32_t prim;
     float b1, b2;
+    uint32_t input, local; // build input and input-local index (AUX only)
 };
 
 // Traverse one ray.  ANY: stop at the first hit.  Counts are optional
 // instrumentation (nodes fetched, triangles tested).
-template <bool ANY, bool COUNT, int BLOCK>
+template <bool ANY, bool COUNT, int BLOCK, bool AUX = false>
 WL_HD TravResult
 trav_run(const WlBvh8Node *__restrict__ nodes,
          const WlTriRecord *__restrict__ tris, const TravRay &r, bool active,
@@ -175,6 +176,8 @@ trav_run(const WlBvh8Node *__restrict__ nodes,
     res.prim = WL_INVALID;
     res.b1 = 0.0f;
     res.b2 = 0.0f;
+    res.input = WL_INVALID;
+    res.local = WL_INVALID;
     int sp = 0;
     uint32_t grp_base = 0, grp_mask = 0;
     uint32_t node = 0;
@@ -292,6 +295,11 @@ trav_run(const WlBvh8Node *__restrict__ nodes,
                     res.prim = prim;
                     res.b1 = b1;
                     res.b2 = b2;
+                    if (AUX)
+                    {
+                        res.input = wl_f2u(w2.z);
+                        res.local = wl_f2u(w2.w);
+                    }
                     done = true;
                     break;
                 }
@@ -301,6 +309,11 @@ trav_run(const WlBvh8Node *__restrict__ nodes,
                     res.prim = prim;
                     res.b1 = b1;
                     res.b2 = b2;
+                    if (AUX)
+                    {
+                        res.input = wl_f2u(w2.z);
+                        res.local = wl_f2u(w2.w);
+                    }
                 }
             }
         }

@@ -258,6 +258,17 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
         delete a;
         return set_error(WL_ERROR_LIMIT_EXCEEDED, err);
     }
+    // record the build input and input-local index of every triangle record
+    for (WlTriRecord &r : host.tris)
+    {
+        auto it = std::upper_bound(a->input_offsets.begin(), a->input_offsets.end(), r.prim);
+        uint32_t in = (uint32_t)(it - a->input_offsets.begin()) - 1;
+        // skip empty inputs that share an offset with the next one
+        while (in + 1 < a->input_offsets.size() && a->input_offsets[in + 1] <= r.prim)
+            ++in;
+        r.input = in;
+        r.local = r.prim - a->input_offsets[in];
+    }
     hipError_t e = hipSetDevice(ctx->device);
     size_t nbytes = host.nodes.size() * sizeof(WlBvh8Node);
     size_t tbytes = std::max<size_t>(1, host.tris.size()) * sizeof(WlTriRecord);
@@ -296,6 +307,7 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     inf.device_bytes = nbytes + tbytes;
     inf.sah_cost = host.sah_cost;
     inf.build_seconds = host.build_seconds;
+    wl::register_accel(a);
     *out = a;
     return WL_SUCCESS;
 }
@@ -304,6 +316,7 @@ wl_status wl_accel_destroy(wl_accel a)
 {
     if (!a)
         return set_error(WL_ERROR_INVALID_VALUE, "accel is NULL");
+    wl::unregister_accel(a);
     hipSetDevice(a->ctx->device);
     hipStreamSynchronize(a->ctx->stream);
     hipFree(a->d_nodes);

@@ -59,6 +59,11 @@ namespace wl
 wl_status set_error(wl_status s, const std::string &msg);
 wl_status hip_error(hipError_t e, const char *what);
 
+// live accels, for resolving LaunchParams `traversable` handles
+void register_accel(wl_accel a);
+void unregister_accel(wl_accel a);
+wl_accel lookup_accel(uint64_t handle);
+
 // kernel launchers (traverse.hip)
 hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
                                 uint32_t n, wl_hit *hits, uint32_t *counts,
