@@ -39,8 +39,18 @@ struct wl_sbt_s
 
 namespace
 {
-std::mutex g_accel_mu;
-std::set<wl_accel> g_accels;
+// Heap objects that are never destroyed: handles may be released from
+// finalizers that run after the library's static destructors.
+std::mutex &accel_mu()
+{
+    static std::mutex *m = new std::mutex;
+    return *m;
+}
+std::set<wl_accel> &accels()
+{
+    static std::set<wl_accel> *s = new std::set<wl_accel>;
+    return *s;
+}
 
 constexpr uint32_t kHeaderMagic = 0x574C4847u; // "WLHG"
 
@@ -115,19 +125,19 @@ namespace wl
 {
 void register_accel(wl_accel a)
 {
-    std::lock_guard<std::mutex> g(g_accel_mu);
-    g_accels.insert(a);
+    std::lock_guard<std::mutex> g(accel_mu());
+    accels().insert(a);
 }
 void unregister_accel(wl_accel a)
 {
-    std::lock_guard<std::mutex> g(g_accel_mu);
-    g_accels.erase(a);
+    std::lock_guard<std::mutex> g(accel_mu());
+    accels().erase(a);
 }
 wl_accel lookup_accel(uint64_t handle)
 {
-    std::lock_guard<std::mutex> g(g_accel_mu);
-    auto it = g_accels.find(reinterpret_cast<wl_accel>(handle));
-    return it == g_accels.end() ? nullptr : *it;
+    std::lock_guard<std::mutex> g(accel_mu());
+    auto it = accels().find(reinterpret_cast<wl_accel>(handle));
+    return it == accels().end() ? nullptr : *it;
 }
 } // namespace wl
 
