@@ -149,8 +149,9 @@ def test_example_pathtracer_launch_matches_oracle(gpu_ctx):
           f"mean {g[:, :3].mean():.6f} vs {o[:, :3].mean():.6f}")
     # a pixel averages 8 paths of up to 10 bounces; one path moved by an ulp
     # of the device sinf/cosf onto another triangle changes the pixel
-    assert frac >= 0.99
-    assert abs(g[:, :3].mean() - o[:, :3].mean()) <= 2e-3 * o[:, :3].mean()
+    # (an emitter of radiance 60 makes such a pixel differ by O(1), so image
+    # means are not a sharp check here; the per-pixel fractions are)
+    assert frac >= 0.99 and exact >= 0.97
     assert (g[:, 3] == 1).all()
     # FromFloatToChar
     u8 = wl.float_to_u8(buf.reshape(-1)).cpu().numpy()

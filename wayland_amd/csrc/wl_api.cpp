@@ -26,6 +26,21 @@ wl_status hip_error(hipError_t e, const char *what)
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? WL_ERROR_OUT_OF_MEMORY : WL_ERROR_DEVICE;
 }
+
+void ctx_retain(wl_context ctx) { ctx->refs.fetch_add(1); }
+
+void ctx_release(wl_context ctx)
+{
+    if (ctx->refs.fetch_sub(1) != 1)
+        return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream)
+    {
+        hipStreamSynchronize(ctx->stream);
+        hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
 } // namespace wl
 
 using wl::set_error;
@@ -98,13 +113,7 @@ wl_status wl_context_destroy(wl_context ctx)
 {
     if (!ctx)
         return set_error(WL_ERROR_INVALID_VALUE, "context is NULL");
-    hipSetDevice(ctx->device);
-    if (ctx->stream)
-    {
-        hipStreamSynchronize(ctx->stream);
-        hipStreamDestroy(ctx->stream);
-    }
-    delete ctx;
+    wl::ctx_release(ctx);
     return WL_SUCCESS;
 }
 
@@ -308,6 +317,7 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     inf.sah_cost = host.sah_cost;
     inf.build_seconds = host.build_seconds;
     wl::register_accel(a);
+    wl::ctx_retain(ctx);
     *out = a;
     return WL_SUCCESS;
 }
@@ -318,10 +328,12 @@ wl_status wl_accel_destroy(wl_accel a)
         return set_error(WL_ERROR_INVALID_VALUE, "accel is NULL");
     wl::unregister_accel(a);
     hipSetDevice(a->ctx->device);
-    hipStreamSynchronize(a->ctx->stream);
+    hipDeviceSynchronize();
     hipFree(a->d_nodes);
     hipFree(a->d_tris);
+    wl_context ctx = a->ctx;
     delete a;
+    wl::ctx_release(ctx);
     return WL_SUCCESS;
 }
 

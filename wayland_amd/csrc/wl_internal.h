@@ -3,19 +3,29 @@
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
+#include <atomic>
 #include <string>
 #include <vector>
 
 #include "../../include/wayland_hip.h"
 #include "bvh8.h"
 
+// Reference counted: the caller's handle plus one reference per accel, scene
+// and pipeline made from it, so children may be released in any order.
 struct wl_context_s
 {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string cache_path;
     int num_cus = 0;
+    std::atomic<int> refs{ 1 };
 };
+
+namespace wl
+{
+void ctx_retain(wl_context ctx);
+void ctx_release(wl_context ctx);
+} // namespace wl
 
 struct wl_accel_s
 {

@@ -238,6 +238,7 @@ wl_status wl_pipeline_create(wl_context ctx, const wl_program_group *groups,
     if (!p)
         return set_error(WL_ERROR_OUT_OF_MEMORY, "pipeline allocation");
     p->ctx = ctx;
+    wl::ctx_retain(ctx);
     p->builtin = which;
     p->groups = g;
     p->max_trace_depth = max_trace_depth;
@@ -251,7 +252,9 @@ wl_status wl_pipeline_destroy(wl_pipeline p)
 {
     if (!p)
         return set_error(WL_ERROR_INVALID_VALUE, "pipeline is NULL");
+    wl_context ctx = p->ctx;
     delete p;
+    wl::ctx_release(ctx);
     return WL_SUCCESS;
 }
 

@@ -193,6 +193,7 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
     if (!s)
         return set_error(WL_ERROR_OUT_OF_MEMORY, "scene allocation");
     s->ctx = ctx;
+    wl::ctx_retain(ctx);
     s->accel = accel;
     s->num_meshes = num_meshes;
     s->num_lights = num_lights;
@@ -241,7 +242,9 @@ wl_status wl_scene_destroy(wl_scene s)
     hipFree(s->d_batch);
     hipFree(s->d_counters);
     hipFree(s->d_stats);
+    wl_context ctx = s->ctx;
     delete s;
+    wl::ctx_release(ctx);
     return WL_SUCCESS;
 }
 
