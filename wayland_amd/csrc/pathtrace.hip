@@ -79,9 +79,10 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
+template <int STACK>
 __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int parity)
 {
-    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kPtBlock];
+    __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -333,9 +334,10 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
 }
 
 // ---- shadow: any hit; unoccluded rays add their NEE contribution ----------
+template <int STACK>
 __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
 {
-    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kPtBlock];
+    __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
@@ -462,15 +464,25 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     const uint32_t npaths = P.npix * P.nsamples;
     const uint32_t blocks_all = (npaths + kPtBlock - 1) / kPtBlock;
     const uint32_t g_gen = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
-    const uint32_t g_trav = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 5);
+    // a BVH at most 16 deep needs at most 15 stack entries per lane
+    const bool shallow = S.bvh_depth <= 16;
+    const uint32_t g_trav = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (shallow ? 8 : 5));
     const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
     hipLaunchKernelGGL(k_raygen, dim3(g_gen), dim3(kPtBlock), 0, st, P, B);
     for (uint32_t b = 0; b < P.max_depth; ++b)
     {
         const int parity = (int)(b & 1u);
-        hipLaunchKernelGGL(k_extend, dim3(g_trav), dim3(kPtBlock), 0, st, S, B, parity);
+        if (shallow)
+            hipLaunchKernelGGL(k_extend<16>, dim3(g_trav), dim3(kPtBlock), 0, st, S, B, parity);
+        else
+            hipLaunchKernelGGL(k_extend<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
+                               S, B, parity);
         hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B, parity);
-        hipLaunchKernelGGL(k_shadow, dim3(g_trav), dim3(kPtBlock), 0, st, S, B);
+        if (shallow)
+            hipLaunchKernelGGL(k_shadow<16>, dim3(g_trav), dim3(kPtBlock), 0, st, S, B);
+        else
+            hipLaunchKernelGGL(k_shadow<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
+                               S, B);
     }
     return hipGetLastError();
 }

@@ -2,6 +2,10 @@
 // One ray per lane, 256 lanes per workgroup, LDS stack per lane (see
 // traverse_device.h).  These are the kernels behind wl_trace_closest and
 // wl_trace_any, which stand in for optixTrace (PathTracing.cu:72-75, :92-102).
+//
+// The LDS stack is sized per launch: a BVH at most 16 levels deep needs at
+// most 15 entries, so its kernels reserve 16 (16 KiB per workgroup) instead
+// of 32, which doubles the workgroups that fit a CU's LDS.
 #include "traverse_device.h"
 #include "wl_internal.h"
 
@@ -10,13 +14,13 @@ namespace wl
 
 constexpr int kTraceBlock = 256;
 
-template <bool COUNT>
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits,
     uint32_t *__restrict__ counts)
 {
-    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kTraceBlock];
+    __shared__ uint32_t stack[STACK * kTraceBlock];
     const uint32_t rid = blockIdx.x * kTraceBlock + threadIdx.x;
     const bool active = rid < n;
     TravRay r;
@@ -49,11 +53,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     }
 }
 
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_any(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
     const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits)
 {
-    __shared__ uint32_t stack[WL_BVH8_STACK_DEPTH * kTraceBlock];
+    __shared__ uint32_t stack[STACK * kTraceBlock];
     const uint32_t rid = blockIdx.x * kTraceBlock + threadIdx.x;
     const bool active = rid < n;
     TravRay r;
@@ -85,6 +90,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_any(
     }
 }
 
+// Stack entries a traversal of this BVH can hold at once: depth - 1.
+static inline bool shallow(const wl_accel_s *a) { return a->info.depth <= 16; }
+
 hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
                                 uint32_t n, wl_hit *hits, uint32_t *counts,
                                 hipStream_t s)
@@ -92,12 +100,25 @@ hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
     if (n == 0)
         return hipSuccess;
     dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
+    const bool sh = shallow(a);
     if (counts)
-        hipLaunchKernelGGL(k_trace_closest<true>, grid, block, 0, s, a->d_nodes,
-                           a->d_tris, rays, n, hits, counts);
+    {
+        if (sh)
+            hipLaunchKernelGGL((k_trace_closest<true, 16>), grid, block, 0, s, a->d_nodes,
+                               a->d_tris, rays, n, hits, counts);
+        else
+            hipLaunchKernelGGL((k_trace_closest<true, WL_BVH8_STACK_DEPTH>), grid, block, 0, s,
+                               a->d_nodes, a->d_tris, rays, n, hits, counts);
+    }
     else
-        hipLaunchKernelGGL(k_trace_closest<false>, grid, block, 0, s, a->d_nodes,
-                           a->d_tris, rays, n, hits, counts);
+    {
+        if (sh)
+            hipLaunchKernelGGL((k_trace_closest<false, 16>), grid, block, 0, s, a->d_nodes,
+                               a->d_tris, rays, n, hits, counts);
+        else
+            hipLaunchKernelGGL((k_trace_closest<false, WL_BVH8_STACK_DEPTH>), grid, block, 0, s,
+                               a->d_nodes, a->d_tris, rays, n, hits, counts);
+    }
     return hipGetLastError();
 }
 
@@ -107,8 +128,12 @@ hipError_t launch_trace_any(const wl_accel_s *a, const wl_ray *rays,
     if (n == 0)
         return hipSuccess;
     dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    hipLaunchKernelGGL(k_trace_any, grid, block, 0, s, a->d_nodes, a->d_tris,
-                       rays, n, bits);
+    if (shallow(a))
+        hipLaunchKernelGGL((k_trace_any<16>), grid, block, 0, s, a->d_nodes, a->d_tris, rays,
+                           n, bits);
+    else
+        hipLaunchKernelGGL((k_trace_any<WL_BVH8_STACK_DEPTH>), grid, block, 0, s, a->d_nodes,
+                           a->d_tris, rays, n, bits);
     return hipGetLastError();
 }
 

@@ -42,6 +42,7 @@ struct PtScene
     const PtMesh *meshes;
     const wl_disney_material *materials;
     const PtLight *lights;
+    uint32_t bvh_depth;
 };
 
 // One batch = pixels [pix0, pix0 + npix) x samples [frame0, frame0 + nsamples)

@@ -76,6 +76,7 @@ wl::PtScene scene_view(wl_scene s)
     S.meshes = (const wl::PtMesh *)s->d_meshes;
     S.materials = s->d_materials;
     S.lights = (const wl::PtLight *)s->d_lights;
+    S.bvh_depth = s->accel->info.depth;
     return S;
 }
 
