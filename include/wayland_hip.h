@@ -3,7 +3,7 @@
  * EasyRender (Extra-Creativity/Wayland).
  *
  * This is the boundary that replaces the OptiX backend
- * (src/Backend/Optix/Core/*) under the EasyRender path tracer.  Every entry
+ * (src/Backend/Optix/Core) under the EasyRender path tracer.  Every entry
  * point names the reference interface it stands in for.  Plain C: opaque
  * handles, plain pointers and sizes; a stream is passed as `void *` holding a
  * hipStream_t (NULL = HIP's null stream, as everywhere in HIP; the context's

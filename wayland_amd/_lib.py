@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libwayland_hip.so")
+# WL_LIBRARY_PATH selects another build of the same ABI (e.g. a tuning build)
+LIB_PATH = os.environ.get("WL_LIBRARY_PATH") or os.path.join(PKG, "libwayland_hip.so")
 
 WL_INVALID_INDEX = 0xFFFFFFFF
 

@@ -23,7 +23,7 @@ static thread_local const char *g_fault_what = nullptr;
         {                                                                    \
             g_faults++;                                                      \
             g_fault_what = what;                                             \
-            return res;                                                      \
+            return true;                                                     \
         }                                                                    \
     } while (0)
 

@@ -22,7 +22,7 @@
 // Every kernel loops over a device-side queue count with a fixed grid, so the
 // host enqueues a whole batch without synchronising.
 #include "shading_device.h"
-#include "traverse_device.h"
+#include "traverse_persistent.h"
 #include "wl_pt.h"
 
 namespace wl
@@ -75,7 +75,10 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
         B.qpath0[p] = p;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
         B.counters[0] = npaths;
+        B.counters[3] = 0; // first extend kernel's ray counter
+    }
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
@@ -88,29 +91,22 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
     {
         B.counters[parity ^ 1] = 0; // next queue
         B.counters[2] = 0;          // shadow queue
+        B.counters[4] = 0;          // shadow kernel's ray counter
         B.stats[0] += count;        // extension rays (single writer)
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
-    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
-    {
-        const uint32_t i = base + threadIdx.x;
-        const bool active = i < count;
-        TravRay r;
-        if (active)
-        {
+    // persistent lanes refill from counters[3] (zeroed by raygen / shade)
+    trav_persistent<false, kPtBlock, false>(
+        S.nodes, S.tris, count, &B.counters[3], stack + threadIdx.x,
+        [&](uint32_t i, TravRay &r) {
             const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
             const float4 a = rp[0], b = rp[1];
             trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-        }
-        else
-            trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
-        uint32_t nn = 0, nt = 0;
-        TravResult h = trav_run<false, false, kPtBlock>(S.nodes, S.tris, r, active,
-                                                       stack + threadIdx.x, nn, nt);
-        if (active)
+        },
+        [&](uint32_t i, const TravResult &h) {
             reinterpret_cast<float4 *>(B.hits)[i] =
                 make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
-    }
+        });
 }
 
 // ---- shade ------------------------------------------------------------------
@@ -118,6 +114,8 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                                                     int parity)
 {
     const uint32_t count = B.counters[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        B.counters[3] = 0; // the next extend kernel's ray counter
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
     uint32_t *qout = parity ? B.qpath0 : B.qpath1;
     wl_ray *rout = parity ? B.rays0 : B.rays1;
@@ -341,24 +339,18 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
-    for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
-    {
-        const uint32_t i = base + threadIdx.x;
-        const bool active = i < count;
-        TravRay r;
-        if (active)
-        {
+    // persistent lanes refill from counters[4] (zeroed by extend)
+    trav_persistent<true, kPtBlock, false>(
+        S.nodes, S.tris, count, &B.counters[4], stack + threadIdx.x,
+        [&](uint32_t i, TravRay &r) {
             const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
             const float4 a = rp[0], b = rp[1];
             trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-        }
-        else
-            trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
-        uint32_t nn = 0, nt = 0;
-        TravResult h = trav_run<true, false, kPtBlock>(S.nodes, S.tris, r, active,
-                                                      stack + threadIdx.x, nn, nt);
-        if (active && h.prim == WL_INVALID)
-        {
+        },
+        [&](uint32_t i, const TravResult &h) {
+            if (h.prim != WL_INVALID)
+                return;
+            // unoccluded: add the NEE contribution (one shadow ray per path)
             const uint32_t path = B.spath[i];
             const float4 c = B.scontrib[i];
             float4 rr = B.rad[path];
@@ -366,8 +358,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
             rr.y = rr.y + c.y;
             rr.z = rr.z + c.z;
             B.rad[path] = rr;
-        }
-    }
+        });
 }
 
 // ---- film -----------------------------------------------------------------
@@ -466,7 +457,20 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     const uint32_t g_gen = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
     // a BVH at most 16 deep needs at most 15 stack entries per lane
     const bool shallow = S.bvh_depth <= 16;
-    const uint32_t g_trav = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (shallow ? 8 : 5));
+    // persistent traversal grids: the workgroups that fit the GPU at once
+    static int occ16 = 0, occ32 = 0;
+    if (!occ16)
+    {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ16, k_extend<16>, kPtBlock, 0) !=
+                hipSuccess || occ16 < 1)
+            occ16 = 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ32, k_extend<WL_BVH8_STACK_DEPTH>,
+                                                         kPtBlock, 0) != hipSuccess ||
+            occ32 < 1)
+            occ32 = 4;
+    }
+    const uint32_t g_trav =
+        std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (uint32_t)(shallow ? occ16 : occ32));
     const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
     hipLaunchKernelGGL(k_raygen, dim3(g_gen), dim3(kPtBlock), 0, st, P, B);
     for (uint32_t b = 0; b < P.max_depth; ++b)

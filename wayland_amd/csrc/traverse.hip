@@ -1,12 +1,14 @@
 // traverse.hip — raw closest-hit / any-hit traversal kernels for gfx950.
-// One ray per lane, 256 lanes per workgroup, LDS stack per lane (see
-// traverse_device.h).  These are the kernels behind wl_trace_closest and
-// wl_trace_any, which stand in for optixTrace (PathTracing.cu:72-75, :92-102).
+// These are the kernels behind wl_trace_closest and wl_trace_any, which stand
+// in for optixTrace (PathTracing.cu:72-75, :92-102).
 //
-// The LDS stack is sized per launch: a BVH at most 16 levels deep needs at
-// most 15 entries, so its kernels reserve 16 (16 KiB per workgroup) instead
-// of 32, which doubles the workgroups that fit a CU's LDS.
-#include "traverse_device.h"
+// One ray per lane, 256 lanes per workgroup, an LDS stack per lane, and a
+// persistent grid whose lanes refill from a global ray counter
+// (traverse_persistent.h).  The LDS stack is sized per launch: a BVH at most
+// 16 levels deep needs at most 15 entries, so its kernels reserve 16
+// (16 KiB per workgroup) instead of 32.  The instrumented closest-hit kernel
+// (node / triangle counts) keeps one ray per lane without refill.
+#include "traverse_persistent.h"
 #include "wl_internal.h"
 
 namespace wl
@@ -14,8 +16,48 @@ namespace wl
 
 constexpr int kTraceBlock = 256;
 
-template <bool COUNT, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
+__device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32_t i, TravRay &r)
+{
+    const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest_p(
+    const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
+    const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits,
+    uint32_t *__restrict__ counter)
+{
+    __shared__ uint32_t stack[STACK * kTraceBlock];
+    trav_persistent<false, kTraceBlock, false>(
+        nodes, tris, n, counter, stack + threadIdx.x,
+        [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
+        [&](uint32_t i, const TravResult &h) {
+            reinterpret_cast<float4 *>(hits)[i] =
+                make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+        });
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void k_trace_any_p(const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
+                   const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits,
+                   uint32_t *__restrict__ counter)
+{
+    __shared__ uint32_t stack[STACK * kTraceBlock];
+    trav_persistent<true, kTraceBlock, false>(
+        nodes, tris, n, counter, stack + threadIdx.x,
+        [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
+        [&](uint32_t i, const TravResult &h) {
+            if (h.prim != WL_INVALID)
+                atomicOr(&bits[i >> 5], 1u << (i & 31u));
+        });
+}
+
+// Instrumented closest hit: one ray per lane, per-ray node/triangle counts.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest_counted(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits,
     uint32_t *__restrict__ counts)
@@ -25,73 +67,85 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const bool active = rid < n;
     TravRay r;
     if (active)
-    {
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + rid);
-        const float4 a = rp[0], b = rp[1];
-        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-    }
+        load_ray(rays, rid, r);
     else
-    {
         trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
-    }
     uint32_t nn = 0, nt = 0;
-    TravResult h = trav_run<false, COUNT, kTraceBlock>(nodes, tris, r, active,
-                                                       stack + threadIdx.x, nn, nt);
+    TravResult h = trav_run<false, true, kTraceBlock>(nodes, tris, r, active,
+                                                     stack + threadIdx.x, nn, nt);
     if (active)
     {
-        float4 o;
-        o.x = h.t;
-        o.y = __uint_as_float(h.prim);
-        o.z = h.b1;
-        o.w = h.b2;
-        reinterpret_cast<float4 *>(hits)[rid] = o;
-        if (COUNT)
-        {
-            counts[2 * (size_t)rid] = nn;
-            counts[2 * (size_t)rid + 1] = nt;
-        }
+        reinterpret_cast<float4 *>(hits)[rid] =
+            make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+        counts[2 * (size_t)rid] = nn;
+        counts[2 * (size_t)rid + 1] = nt;
     }
+}
+
+// Zeroes the ray counter and, for any-hit, the occlusion words.
+__global__ void k_trace_prologue(uint32_t *counter, uint32_t *words, uint32_t nwords)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+        counter[0] = 0;
+    for (uint32_t k = i; k < nwords; k += gridDim.x * blockDim.x)
+        words[k] = 0;
+}
+
+static inline bool shallow(const wl_accel_s *a) { return a->info.depth <= 16; }
+
+// Persistent grid: the workgroups that fit the GPU at once (the launch then
+// takes at most one workgroup per 256 rays).
+template <class K>
+static uint32_t persistent_grid(K kernel, uint32_t /*unused*/, int num_cus)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 4;
+    return (uint32_t)(per_cu * std::max(1, num_cus));
 }
 
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_any(
-    const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
-    const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits)
+static hipError_t closest_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t n,
+                               wl_hit *hits, hipStream_t s)
 {
-    __shared__ uint32_t stack[STACK * kTraceBlock];
-    const uint32_t rid = blockIdx.x * kTraceBlock + threadIdx.x;
-    const bool active = rid < n;
-    TravRay r;
-    if (active)
+    static uint32_t grid_cache = 0; // depends only on the kernel and the device
+    static int cus_cache = -1;
+    if (cus_cache != a->ctx->num_cus)
     {
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + rid);
-        const float4 a = rp[0], b = rp[1];
-        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+        grid_cache = persistent_grid(k_trace_closest_p<STACK>, 0xFFFFFFFFu, a->ctx->num_cus);
+        cus_cache = a->ctx->num_cus;
     }
-    else
-    {
-        trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
-    }
-    uint32_t nn = 0, nt = 0;
-    TravResult h = trav_run<true, false, kTraceBlock>(nodes, tris, r, active,
-                                                      stack + threadIdx.x, nn, nt);
-    const bool occ = active && h.prim != WL_INVALID;
-    // wave64 ballot: one 64-bit word of occlusion bits per wave
-    const uint64_t m = __ballot(occ);
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave_first = rid - lane;
-    if (lane == 0)
-    {
-        const uint32_t w0 = wave_first >> 5;
-        if ((size_t)w0 * 32 < n)
-            bits[w0] = (uint32_t)m;
-        if ((size_t)(w0 + 1) * 32 < n)
-            bits[w0 + 1] = (uint32_t)(m >> 32);
-    }
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(
+        grid_cache, ((uint64_t)n + kTraceBlock - 1) / kTraceBlock);
+    hipLaunchKernelGGL(k_trace_prologue, dim3(1), dim3(64), 0, s, a->d_counter, nullptr, 0u);
+    hipLaunchKernelGGL(k_trace_closest_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s,
+                       a->d_nodes, a->d_tris, rays, n, hits, a->d_counter);
+    return hipGetLastError();
 }
 
-// Stack entries a traversal of this BVH can hold at once: depth - 1.
-static inline bool shallow(const wl_accel_s *a) { return a->info.depth <= 16; }
+template <int STACK>
+static hipError_t any_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t n,
+                           uint32_t *bits, hipStream_t s)
+{
+    static uint32_t grid_cache = 0;
+    static int cus_cache = -1;
+    if (cus_cache != a->ctx->num_cus)
+    {
+        grid_cache = persistent_grid(k_trace_any_p<STACK>, 0xFFFFFFFFu, a->ctx->num_cus);
+        cus_cache = a->ctx->num_cus;
+    }
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(
+        grid_cache, ((uint64_t)n + kTraceBlock - 1) / kTraceBlock);
+    const uint32_t words = (n + 31) / 32;
+    hipLaunchKernelGGL(k_trace_prologue, dim3(std::min<uint32_t>((words + 255) / 256 + 1, 4096)),
+                       dim3(256), 0, s, a->d_counter, bits, words);
+    hipLaunchKernelGGL(k_trace_any_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s, a->d_nodes,
+                       a->d_tris, rays, n, bits, a->d_counter);
+    return hipGetLastError();
+}
 
 hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
                                 uint32_t n, wl_hit *hits, uint32_t *counts,
@@ -99,27 +153,20 @@ hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
 {
     if (n == 0)
         return hipSuccess;
-    dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
     const bool sh = shallow(a);
     if (counts)
     {
+        dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
         if (sh)
-            hipLaunchKernelGGL((k_trace_closest<true, 16>), grid, block, 0, s, a->d_nodes,
+            hipLaunchKernelGGL(k_trace_closest_counted<16>, grid, block, 0, s, a->d_nodes,
                                a->d_tris, rays, n, hits, counts);
         else
-            hipLaunchKernelGGL((k_trace_closest<true, WL_BVH8_STACK_DEPTH>), grid, block, 0, s,
+            hipLaunchKernelGGL(k_trace_closest_counted<WL_BVH8_STACK_DEPTH>, grid, block, 0, s,
                                a->d_nodes, a->d_tris, rays, n, hits, counts);
+        return hipGetLastError();
     }
-    else
-    {
-        if (sh)
-            hipLaunchKernelGGL((k_trace_closest<false, 16>), grid, block, 0, s, a->d_nodes,
-                               a->d_tris, rays, n, hits, counts);
-        else
-            hipLaunchKernelGGL((k_trace_closest<false, WL_BVH8_STACK_DEPTH>), grid, block, 0, s,
-                               a->d_nodes, a->d_tris, rays, n, hits, counts);
-    }
-    return hipGetLastError();
+    return sh ? closest_impl<16>(a, rays, n, hits, s)
+              : closest_impl<WL_BVH8_STACK_DEPTH>(a, rays, n, hits, s);
 }
 
 hipError_t launch_trace_any(const wl_accel_s *a, const wl_ray *rays,
@@ -127,14 +174,8 @@ hipError_t launch_trace_any(const wl_accel_s *a, const wl_ray *rays,
 {
     if (n == 0)
         return hipSuccess;
-    dim3 grid((n + kTraceBlock - 1) / kTraceBlock), block(kTraceBlock);
-    if (shallow(a))
-        hipLaunchKernelGGL((k_trace_any<16>), grid, block, 0, s, a->d_nodes, a->d_tris, rays,
-                           n, bits);
-    else
-        hipLaunchKernelGGL((k_trace_any<WL_BVH8_STACK_DEPTH>), grid, block, 0, s, a->d_nodes,
-                           a->d_tris, rays, n, bits);
-    return hipGetLastError();
+    return shallow(a) ? any_impl<16>(a, rays, n, bits, s)
+                      : any_impl<WL_BVH8_STACK_DEPTH>(a, rays, n, bits, s);
 }
 
 } // namespace wl

@@ -163,186 +163,206 @@ struct TravResult
     uint32_t input, local; // build input and input-local index (AUX only)
 };
 
-// Traverse one ray.  ANY: stop at the first hit.  Counts are optional
-// instrumentation (nodes fetched, triangles tested).
+// Resumable traversal state of one ray: the current node group (child_base,
+// hit mask in key space), the node to visit next, the stack depth and the
+// best hit so far.  The stack itself lives in LDS.
+struct TravState
+{
+    TravResult res;
+    uint32_t grp_base, grp_mask, node;
+    int sp;
+    bool visit;
+};
+
+WL_HD void trav_init(TravState &st, const TravRay &r, bool active)
+{
+    st.res.t = r.tmax;
+    st.res.prim = WL_INVALID;
+    st.res.b1 = 0.0f;
+    st.res.b2 = 0.0f;
+    st.res.input = WL_INVALID;
+    st.res.local = WL_INVALID;
+    st.sp = 0;
+    st.grp_base = 0;
+    st.grp_mask = 0;
+    st.node = 0;
+    st.visit = active;
+}
+
+// One step: visit the pending node (8 child boxes), test the triangles of its
+// leaves, then pick the next node from the current group or the stack.
+// Returns true when the ray is finished (stack empty, or first hit for ANY).
+template <bool ANY, bool COUNT, int BLOCK, bool AUX = false>
+WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
+                     const WlTriRecord *__restrict__ tris, const TravRay &r,
+                     TravState &st, uint32_t *lds_stack_lane, uint32_t &n_nodes,
+                     uint32_t &n_tris)
+{
+    uint32_t trimask = 0, tri_base = 0;
+    const float tmin = r.tmin;
+    if (st.visit)
+    {
+        const uint32_t node = st.node;
+        WL_TRAV_CHECK(node < wl_trav_nodes_bound, "node index");
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
+        const uint4 h0 = np[0];
+        const uint4 h1 = np[1];
+        const uint4 qlx = np[2], qly = np[3], qlz = np[4];
+        const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
+        if (COUNT)
+            ++n_nodes;
+        const float px = wl_u2f(h0.x), py = wl_u2f(h0.y), pz = wl_u2f(h0.z);
+        const uint32_t ew = h0.w;
+        const float sx = wl_u2f((ew & 0xFFu) << 23);
+        const float sy = wl_u2f(((ew >> 8) & 0xFFu) << 23);
+        const float sz = wl_u2f(((ew >> 16) & 0xFFu) << 23);
+        const uint32_t imask = ew >> 24;
+        const uint32_t child_base = h1.x;
+        tri_base = h1.y;
+        const uint32_t meta_lo = h1.z, meta_hi = h1.w;
+        // per-axis plane parameters: t = q * sinv + adj
+        const float ax = (px - r.o[0]) * r.inv[0];
+        const float ay = (py - r.o[1]) * r.inv[1];
+        const float az = (pz - r.o[2]) * r.inv[2];
+        const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
+        // near/far quantised planes chosen by the ray's direction signs
+        const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+        const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
+        const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
+        const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
+        const float cull = st.res.t * 1.00001f;
+        const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
+        const uint32_t fxw[4] = { fqx.x, fqx.y, fqx.z, fqx.w };
+        const uint32_t nyw[4] = { nqy.x, nqy.y, nqy.z, nqy.w };
+        const uint32_t fyw[4] = { fqy.x, fqy.y, fqy.z, fqy.w };
+        const uint32_t nzw[4] = { nqz.x, nqz.y, nqz.z, nqz.w };
+        const uint32_t fzw[4] = { fqz.x, fqz.y, fqz.z, fqz.w };
+        uint32_t inner_hits = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+        {
+            const int wi = s >> 1;
+            float qnx, qfx, qny, qfy, qnz, qfz;
+            if (s & 1)
+            {
+                qnx = q_hi16(nxw[wi]); qfx = q_hi16(fxw[wi]);
+                qny = q_hi16(nyw[wi]); qfy = q_hi16(fyw[wi]);
+                qnz = q_hi16(nzw[wi]); qfz = q_hi16(fzw[wi]);
+            }
+            else
+            {
+                qnx = q_lo16(nxw[wi]); qfx = q_lo16(fxw[wi]);
+                qny = q_lo16(nyw[wi]); qfy = q_lo16(fyw[wi]);
+                qnz = q_lo16(nzw[wi]); qfz = q_lo16(fzw[wi]);
+            }
+            const float tnx = __builtin_fmaf(qnx, six, ax);
+            const float tfx = __builtin_fmaf(qfx, six, ax);
+            const float tny = __builtin_fmaf(qny, siy, ay);
+            const float tfy = __builtin_fmaf(qfy, siy, ay);
+            const float tnz = __builtin_fmaf(qnz, siz, az);
+            const float tfz = __builtin_fmaf(qfz, siz, az);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+            float tf = fminf(fminf(tfx, tfy), tfz);
+            tf = __builtin_fmaf(fabsf(tf), 4e-6f, tf);
+            const bool hit = tn <= fminf(tf, cull);
+            if (hit)
+            {
+                if (imask & (1u << s))
+                {
+                    inner_hits |= 1u << (s ^ r.xkey);
+                }
+                else
+                {
+                    const uint32_t m = ((s < 4 ? meta_lo : meta_hi) >> (8 * (s & 3))) & 0xFFu;
+                    // 0xFF marks an empty slot (its inverted box can still
+                    // pass the padded test on a tiny node)
+                    if (m != 0xFFu)
+                    {
+                        const uint32_t off = m >> 2, cnt = (m & 3u) + 1u;
+                        trimask |= ((1u << cnt) - 1u) << off;
+                    }
+                }
+            }
+        }
+        st.grp_base = child_base;
+        st.grp_mask = inner_hits;
+        st.visit = false;
+    }
+    // ---- triangles of the node just visited
+    while (trimask)
+    {
+        const uint32_t i = __builtin_ctz(trimask);
+        trimask &= trimask - 1u;
+        WL_TRAV_CHECK((size_t)tri_base + i < wl_trav_tris_bound, "triangle index");
+        const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
+        const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+        if (COUNT)
+            ++n_tris;
+        float t, b1, b2;
+        if (tri_test(r, w0, w1, w2, t, b1, b2))
+        {
+            const uint32_t prim = wl_f2u(w2.y);
+            if (ANY || t < st.res.t || (t == st.res.t && prim < st.res.prim))
+            {
+                st.res.t = t;
+                st.res.prim = prim;
+                st.res.b1 = b1;
+                st.res.b2 = b2;
+                if (AUX)
+                {
+                    st.res.input = wl_f2u(w2.z);
+                    st.res.local = wl_f2u(w2.w);
+                }
+                if (ANY)
+                    return true;
+            }
+        }
+    }
+    // ---- next node: from the current group, else from the stack
+    if (st.grp_mask == 0)
+    {
+        if (st.sp == 0)
+            return true;
+        --st.sp;
+        const uint32_t e = lds_stack_lane[st.sp * BLOCK];
+        st.grp_base = (e >> 8) << 3;
+        st.grp_mask = e & 0xFFu;
+    }
+    const uint32_t pos = __builtin_ctz(st.grp_mask);
+    st.grp_mask &= st.grp_mask - 1u;
+    st.node = st.grp_base + (pos ^ r.xkey);
+    if (st.grp_mask)
+    {
+        WL_TRAV_CHECK(st.sp < WL_BVH8_STACK_DEPTH, "stack depth");
+        lds_stack_lane[st.sp * BLOCK] = ((st.grp_base >> 3) << 8) | st.grp_mask;
+        ++st.sp;
+    }
+    st.visit = true;
+    return false;
+}
+
+WL_HD void trav_finish(TravResult &res)
+{
+    if (res.prim == WL_INVALID)
+        res.t = wl_u2f(0x7F800000u); // +inf
+}
+
+// Traverse one ray to completion.
 template <bool ANY, bool COUNT, int BLOCK, bool AUX = false>
 WL_HD TravResult
 trav_run(const WlBvh8Node *__restrict__ nodes,
          const WlTriRecord *__restrict__ tris, const TravRay &r, bool active,
          uint32_t *lds_stack_lane, uint32_t &n_nodes, uint32_t &n_tris)
 {
-    TravResult res;
-    res.t = r.tmax;
-    res.prim = WL_INVALID;
-    res.b1 = 0.0f;
-    res.b2 = 0.0f;
-    res.input = WL_INVALID;
-    res.local = WL_INVALID;
-    int sp = 0;
-    uint32_t grp_base = 0, grp_mask = 0;
-    uint32_t node = 0;
-    bool visit = active;
-    const float tmin = r.tmin;
-    for (;;)
-    {
-        uint32_t trimask = 0, tri_base = 0;
-        if (visit)
+    TravState st;
+    trav_init(st, r, active);
+    if (active)
+        while (!trav_step<ANY, COUNT, BLOCK, AUX>(nodes, tris, r, st, lds_stack_lane,
+                                                 n_nodes, n_tris))
         {
-            WL_TRAV_CHECK(node < wl_trav_nodes_bound, "node index");
-            const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
-            const uint4 h0 = np[0];
-            const uint4 h1 = np[1];
-            const uint4 qlx = np[2], qly = np[3], qlz = np[4];
-            const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
-            if (COUNT)
-                ++n_nodes;
-            const float px = wl_u2f(h0.x), py = wl_u2f(h0.y),
-                        pz = wl_u2f(h0.z);
-            const uint32_t ew = h0.w;
-            const float sx = wl_u2f((ew & 0xFFu) << 23);
-            const float sy = wl_u2f(((ew >> 8) & 0xFFu) << 23);
-            const float sz = wl_u2f(((ew >> 16) & 0xFFu) << 23);
-            const uint32_t imask = ew >> 24;
-            const uint32_t child_base = h1.x;
-            tri_base = h1.y;
-            const uint32_t meta_lo = h1.z, meta_hi = h1.w;
-            // per-axis plane parameters: t = q * sinv + adj
-            const float ax = (px - r.o[0]) * r.inv[0];
-            const float ay = (py - r.o[1]) * r.inv[1];
-            const float az = (pz - r.o[2]) * r.inv[2];
-            const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
-            // near/far quantised planes chosen by the ray's direction signs
-            const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
-            const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
-            const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
-            const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
-            const float cull = res.t * 1.00001f;
-            const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
-            const uint32_t fxw[4] = { fqx.x, fqx.y, fqx.z, fqx.w };
-            const uint32_t nyw[4] = { nqy.x, nqy.y, nqy.z, nqy.w };
-            const uint32_t fyw[4] = { fqy.x, fqy.y, fqy.z, fqy.w };
-            const uint32_t nzw[4] = { nqz.x, nqz.y, nqz.z, nqz.w };
-            const uint32_t fzw[4] = { fqz.x, fqz.y, fqz.z, fqz.w };
-            uint32_t inner_hits = 0;
-#pragma unroll
-            for (int s = 0; s < 8; ++s)
-            {
-                const int wi = s >> 1;
-                float qnx, qfx, qny, qfy, qnz, qfz;
-                if (s & 1)
-                {
-                    qnx = q_hi16(nxw[wi]); qfx = q_hi16(fxw[wi]);
-                    qny = q_hi16(nyw[wi]); qfy = q_hi16(fyw[wi]);
-                    qnz = q_hi16(nzw[wi]); qfz = q_hi16(fzw[wi]);
-                }
-                else
-                {
-                    qnx = q_lo16(nxw[wi]); qfx = q_lo16(fxw[wi]);
-                    qny = q_lo16(nyw[wi]); qfy = q_lo16(fyw[wi]);
-                    qnz = q_lo16(nzw[wi]); qfz = q_lo16(fzw[wi]);
-                }
-                const float tnx = __builtin_fmaf(qnx, six, ax);
-                const float tfx = __builtin_fmaf(qfx, six, ax);
-                const float tny = __builtin_fmaf(qny, siy, ay);
-                const float tfy = __builtin_fmaf(qfy, siy, ay);
-                const float tnz = __builtin_fmaf(qnz, siz, az);
-                const float tfz = __builtin_fmaf(qfz, siz, az);
-                const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-                float tf = fminf(fminf(tfx, tfy), tfz);
-                tf = __builtin_fmaf(fabsf(tf), 4e-6f, tf);
-                const bool hit = tn <= fminf(tf, cull);
-                if (hit)
-                {
-                    if (imask & (1u << s))
-                    {
-                        inner_hits |= 1u << (s ^ r.xkey);
-                    }
-                    else
-                    {
-                        const uint32_t m = ((s < 4 ? meta_lo : meta_hi) >> (8 * (s & 3))) & 0xFFu;
-                        // 0xFF marks an empty slot (its inverted box can
-                        // still pass the padded test on a tiny node)
-                        if (m != 0xFFu)
-                        {
-                            const uint32_t off = m >> 2, cnt = (m & 3u) + 1u;
-                            trimask |= ((1u << cnt) - 1u) << off;
-                        }
-                    }
-                }
-            }
-            grp_base = child_base;
-            grp_mask = inner_hits;
-            visit = false;
         }
-        // ---- triangles of the node just visited
-        bool done = false;
-        while (trimask)
-        {
-            const uint32_t i = __builtin_ctz(trimask);
-            trimask &= trimask - 1u;
-            WL_TRAV_CHECK((size_t)tri_base + i < wl_trav_tris_bound, "triangle index");
-            const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
-            const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
-            if (COUNT)
-                ++n_tris;
-            float t, b1, b2;
-            if (tri_test(r, w0, w1, w2, t, b1, b2))
-            {
-                const uint32_t prim = wl_f2u(w2.y);
-                if (ANY)
-                {
-                    res.t = t;
-                    res.prim = prim;
-                    res.b1 = b1;
-                    res.b2 = b2;
-                    if (AUX)
-                    {
-                        res.input = wl_f2u(w2.z);
-                        res.local = wl_f2u(w2.w);
-                    }
-                    done = true;
-                    break;
-                }
-                if (t < res.t || (t == res.t && prim < res.prim))
-                {
-                    res.t = t;
-                    res.prim = prim;
-                    res.b1 = b1;
-                    res.b2 = b2;
-                    if (AUX)
-                    {
-                        res.input = wl_f2u(w2.z);
-                        res.local = wl_f2u(w2.w);
-                    }
-                }
-            }
-        }
-        if (ANY && done)
-            break;
-        // ---- next node: from the current group, else from the stack
-        if (grp_mask == 0)
-        {
-            if (sp == 0)
-                break;
-            --sp;
-            const uint32_t e = lds_stack_lane[sp * BLOCK];
-            grp_base = (e >> 8) << 3;
-            grp_mask = e & 0xFFu;
-        }
-        const uint32_t pos = __builtin_ctz(grp_mask);
-        grp_mask &= grp_mask - 1u;
-        node = grp_base + (pos ^ r.xkey);
-        if (grp_mask)
-        {
-            WL_TRAV_CHECK(sp < WL_BVH8_STACK_DEPTH, "stack depth");
-            lds_stack_lane[sp * BLOCK] = ((grp_base >> 3) << 8) | grp_mask;
-            ++sp;
-        }
-        visit = true;
-    }
-    if (res.prim == WL_INVALID)
-        res.t = wl_u2f(0x7F800000u); // +inf
-    return res;
+    trav_finish(st.res);
+    return st.res;
 }
 
 } // namespace wl

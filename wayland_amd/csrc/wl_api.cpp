@@ -286,6 +286,8 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     if (e == hipSuccess)
         e = hipMalloc(&a->d_tris, tbytes);
     if (e == hipSuccess)
+        e = hipMalloc(&a->d_counter, 256);
+    if (e == hipSuccess)
         e = hipMemcpyAsync(a->d_nodes, host.nodes.data(), nbytes,
                            hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && !host.tris.empty())
@@ -300,6 +302,8 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
             hipFree(a->d_nodes);
         if (a->d_tris)
             hipFree(a->d_tris);
+        if (a->d_counter)
+            hipFree(a->d_counter);
         delete a;
         return wl::hip_error(e, "accel upload");
     }
@@ -331,6 +335,7 @@ wl_status wl_accel_destroy(wl_accel a)
     hipDeviceSynchronize();
     hipFree(a->d_nodes);
     hipFree(a->d_tris);
+    hipFree(a->d_counter);
     wl_context ctx = a->ctx;
     delete a;
     wl::ctx_release(ctx);

@@ -32,6 +32,8 @@ struct wl_accel_s
     wl_context ctx = nullptr;
     WlBvh8Node *d_nodes = nullptr;
     WlTriRecord *d_tris = nullptr;
+    uint32_t *d_counter = nullptr; // ray counter of the persistent trace kernels
+    int occ_closest = 0, occ_any = 0; // resident workgroups per CU (queried once)
     uint32_t num_nodes = 0;
     uint64_t num_tris = 0;
     std::vector<uint32_t> input_offsets;
