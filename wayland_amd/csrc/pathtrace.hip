@@ -75,10 +75,7 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
         B.qpath0[p] = p;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
-    {
         B.counters[0] = npaths;
-        B.counters[3] = 0; // first extend kernel's ray counter
-    }
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
@@ -91,13 +88,11 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
     {
         B.counters[parity ^ 1] = 0; // next queue
         B.counters[2] = 0;          // shadow queue
-        B.counters[4] = 0;          // shadow kernel's ray counter
         B.stats[0] += count;        // extension rays (single writer)
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
-    // persistent lanes refill from counters[3] (zeroed by raygen / shade)
     trav_persistent<false, kPtBlock, false>(
-        S.nodes, S.tris, count, &B.counters[3], stack + threadIdx.x,
+        S.nodes, S.tris, count, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) {
             const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
             const float4 a = rp[0], b = rp[1];
@@ -114,8 +109,6 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                                                     int parity)
 {
     const uint32_t count = B.counters[parity];
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        B.counters[3] = 0; // the next extend kernel's ray counter
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
     uint32_t *qout = parity ? B.qpath0 : B.qpath1;
     wl_ray *rout = parity ? B.rays0 : B.rays1;
@@ -339,9 +332,8 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
-    // persistent lanes refill from counters[4] (zeroed by extend)
     trav_persistent<true, kPtBlock, false>(
-        S.nodes, S.tris, count, &B.counters[4], stack + threadIdx.x,
+        S.nodes, S.tris, count, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) {
             const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
             const float4 a = rp[0], b = rp[1];

@@ -3,7 +3,7 @@
 // in for optixTrace (PathTracing.cu:72-75, :92-102).
 //
 // One ray per lane, 256 lanes per workgroup, an LDS stack per lane, and a
-// persistent grid whose lanes refill from a global ray counter
+// persistent grid whose lanes refill from their wave's range of rays
 // (traverse_persistent.h).  The LDS stack is sized per launch: a BVH at most
 // 16 levels deep needs at most 15 entries, so its kernels reserve 16
 // (16 KiB per workgroup) instead of 32.  The instrumented closest-hit kernel
@@ -26,12 +26,11 @@ __device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest_p(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
-    const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits,
-    uint32_t *__restrict__ counter)
+    const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
     trav_persistent<false, kTraceBlock, false>(
-        nodes, tris, n, counter, stack + threadIdx.x,
+        nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {
             reinterpret_cast<float4 *>(hits)[i] =
@@ -42,12 +41,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest_p(
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 void k_trace_any_p(const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
-                   const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits,
-                   uint32_t *__restrict__ counter)
+                   const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
     trav_persistent<true, kTraceBlock, false>(
-        nodes, tris, n, counter, stack + threadIdx.x,
+        nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {
             if (h.prim != WL_INVALID)
@@ -82,12 +80,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest_counted(
     }
 }
 
-// Zeroes the ray counter and, for any-hit, the occlusion words.
-__global__ void k_trace_prologue(uint32_t *counter, uint32_t *words, uint32_t nwords)
+// Zeroes the any-hit occlusion words (the kernel ORs bits in).
+__global__ void k_zero_words(uint32_t *words, uint32_t nwords)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0)
-        counter[0] = 0;
     for (uint32_t k = i; k < nwords; k += gridDim.x * blockDim.x)
         words[k] = 0;
 }
@@ -120,9 +116,8 @@ static hipError_t closest_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t
     }
     const uint32_t grid = (uint32_t)std::min<uint64_t>(
         grid_cache, ((uint64_t)n + kTraceBlock - 1) / kTraceBlock);
-    hipLaunchKernelGGL(k_trace_prologue, dim3(1), dim3(64), 0, s, a->d_counter, nullptr, 0u);
     hipLaunchKernelGGL(k_trace_closest_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s,
-                       a->d_nodes, a->d_tris, rays, n, hits, a->d_counter);
+                       a->d_nodes, a->d_tris, rays, n, hits);
     return hipGetLastError();
 }
 
@@ -140,10 +135,10 @@ static hipError_t any_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t n,
     const uint32_t grid = (uint32_t)std::min<uint64_t>(
         grid_cache, ((uint64_t)n + kTraceBlock - 1) / kTraceBlock);
     const uint32_t words = (n + 31) / 32;
-    hipLaunchKernelGGL(k_trace_prologue, dim3(std::min<uint32_t>((words + 255) / 256 + 1, 4096)),
-                       dim3(256), 0, s, a->d_counter, bits, words);
+    hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>((words + 255) / 256, 4096)),
+                       dim3(256), 0, s, bits, words);
     hipLaunchKernelGGL(k_trace_any_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s, a->d_nodes,
-                       a->d_tris, rays, n, bits, a->d_counter);
+                       a->d_tris, rays, n, bits);
     return hipGetLastError();
 }
 

@@ -4,11 +4,15 @@
 // that runs one ray per lane to completion idles most of its lanes near the
 // end.  Here a fixed grid of waves loops: every lane advances its ray by one
 // node (trav_step); once at least kRefillLanes lanes of a wave are idle, the
-// wave takes that many new ray indices from a global counter with one atomic
-// (ballot + mbcnt prefix) and the idle lanes start them (after Aila & Laine,
-// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009).
-// Every wave leaves the loop once the counter has passed n and none of its
-// lanes holds a ray, so the grid always drains.
+// idle lanes start the next rays of the wave's own contiguous range of ray
+// indices (after Aila & Laine, "Understanding the Efficiency of Ray Traversal
+// on GPUs", HPG 2009).
+//
+// Each wave owns ceil(n / waves) consecutive rays and refills from a
+// wave-uniform register, so no atomics are needed; with >1000 random rays per
+// wave, the per-wave totals differ by a few percent at most.  A wave leaves
+// the loop when its range is used up and none of its lanes holds a ray, so
+// the grid always drains.
 #pragma once
 #include "traverse_device.h"
 
@@ -24,50 +28,49 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 }
 
 // fetch(i, TravRay&) sets up ray i (i < n); emit(i, const TravResult&)
-// consumes its result.  `counter` must be zero when the grid starts.
+// consumes its result.  BLOCK must be a multiple of 64.
 template <bool ANY, int BLOCK, bool AUX, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
                                                 const WlTriRecord *__restrict__ tris,
-                                                uint32_t n, uint32_t *counter,
-                                                uint32_t *stack_lane, Fetch fetch,
-                                                Emit emit)
+                                                uint32_t n, uint32_t *stack_lane,
+                                                Fetch fetch, Emit emit)
 {
+    static_assert(BLOCK % 64 == 0, "whole waves only");
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64;
+    const uint32_t chunk = (uint32_t)(((uint64_t)n + waves - 1) / waves);
+    const uint64_t begin64 = (uint64_t)wave * chunk;
+    const uint32_t end = (uint32_t)(begin64 + chunk < n ? begin64 + chunk : n);
+    uint32_t next = (uint32_t)(begin64 < n ? begin64 : n); // wave-uniform
     TravRay r;
     trav_setup(r, 0, 0, 0, 1, 0, 0, 0, 0);
     TravState st;
     trav_init(st, r, false);
     uint32_t rid = 0;
     bool alive = false;
-    bool exhausted = false; // wave-uniform
-    const uint32_t lane = __lane_id();
     uint32_t nn = 0, nt = 0;
     for (;;)
     {
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (!exhausted && nidle >= (uint32_t)kRefillLanes)
+        if (next < end && nidle >= (uint32_t)kRefillLanes)
         {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader)
-                base = atomicAdd(counter, nidle);
-            base = __shfl(base, leader);
-            if ((uint64_t)base + nidle >= n)
-                exhausted = true;
             if (!alive)
             {
-                rid = base + lane_prefix(idle);
-                if (rid < n)
+                const uint32_t k = next + lane_prefix(idle);
+                if (k < end)
                 {
+                    rid = k;
                     fetch(rid, r);
                     trav_init(st, r, true);
                     alive = true;
                 }
             }
+            next = next + nidle < end ? next + nidle : end;
         }
         if (__ballot(alive) == 0)
         {
-            if (exhausted)
+            if (next >= end)
                 break;
             continue;
         }
