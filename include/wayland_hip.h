@@ -52,6 +52,7 @@ typedef enum wl_status
 typedef struct wl_context_s *wl_context;
 typedef struct wl_accel_s *wl_accel;
 typedef struct wl_scene_s *wl_scene;
+typedef struct wl_guide_s *wl_guide;
 
 /* ---- errors (reference: HostUtils/ErrorCheck.h:9-58, ErrorCheck.cpp:14-22) */
 const char *wl_status_string(wl_status status);
@@ -244,6 +245,80 @@ wl_status wl_pt_launch_frame(wl_scene scene, const wl_pt_config *cfg,
 /* Film resolve: d_rgb[i] = sum / nsamples (+ compensation). */
 wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
                           uint32_t nsamples, float *d_rgb, void *stream);
+
+/* ---- learned path guiding -------------------------------------------------
+ * No counterpart in the reference, whose path tracer samples the BSDF only
+ * (Sample.h:134-231).  A small MLP (64 -> 4 x 64 ReLU -> 32, BF16 MFMA
+ * forward and backward, FP32 accumulation and FP32 master weights) maps a
+ * query (position normalised to the scene box, normal, outgoing direction;
+ * sin/cos frequency encoding of the position) to an 8-lobe von Mises-Fisher
+ * mixture over directions.  With a guide attached to a scene the tracer picks
+ * the guide with probability mix_alpha at vertices whose material has
+ * trans = 0 and clearcoat = 0, else the reference BSDF sampler, and weights
+ * by the mixed density (one-sample MIS), so the estimate stays unbiased.  In
+ * training mode every path records its first guided vertex, and after each
+ * batch the recorded samples train the network for steps_per_batch Adam
+ * steps on the KL objective -F/p log q (Mueller et al., TOG 2019).
+ *
+ * Parameter layout (WL_GUIDE_PARAMS floats): for layer l = 0..4 the weights
+ * W_l[out][in] row-major then the biases b_l[out]; out = 64 (l < 4) or 32.
+ * Output row r: lobe 2 ((r % 16) / 4) + r / 16, parameter r % 4 = logit,
+ * raw concentration (kappa = softplus), theta, phi.
+ * Query: 9 floats.  Training sample: 16 floats = query, sampled direction,
+ * weight F/p, 3 unused. */
+#define WL_GUIDE_PARAMS 18720u
+enum
+{
+    WL_GUIDE_SAMPLE = 1u << 0, /* guided sampling                             */
+    WL_GUIDE_TRAIN = 1u << 1   /* record training samples and train per batch */
+};
+
+typedef struct wl_guide_config
+{
+    uint32_t seed;            /* weight initialisation                        */
+    uint32_t batch_size;      /* samples per Adam step, multiple of 32        */
+    uint32_t steps_per_batch; /* Adam steps after each rendered batch         */
+    uint32_t max_samples;     /* training samples kept per batch              */
+    float learning_rate, beta1, beta2, epsilon;
+    float mix_alpha;          /* probability of sampling the guide            */
+    float max_weight;         /* training weights are clamped to this         */
+    uint32_t pad_[2];
+} wl_guide_config;
+
+typedef struct wl_guide_info
+{
+    uint64_t steps;           /* Adam steps taken                             */
+    uint32_t last_samples;    /* training samples of the last batch           */
+    uint32_t pad_;
+} wl_guide_info;
+
+wl_status wl_guide_default_config(wl_guide_config *cfg);
+wl_status wl_guide_create(wl_context ctx, const wl_guide_config *cfg, wl_guide *out);
+wl_status wl_guide_destroy(wl_guide guide);
+/* Host copies of the FP32 master parameters; set also clears the Adam
+ * state and the step count.  n must be WL_GUIDE_PARAMS. */
+wl_status wl_guide_set_params(wl_guide guide, const float *params, uint32_t n);
+wl_status wl_guide_get_params(wl_guide guide, float *params, uint32_t n);
+wl_status wl_guide_set_mix(wl_guide guide, float mix_alpha);
+/* Synchronises the guide's last stream. */
+wl_status wl_guide_get_info(wl_guide guide, wl_guide_info *info);
+/* Network outputs of n queries: d_out = 32 floats per query (row order). */
+wl_status wl_guide_infer(wl_guide guide, const float *d_queries, uint32_t n, float *d_out,
+                         void *stream);
+/* Mixture sampling on given outputs: d_u = 3 uniforms per query (lobe, then
+ * the two direction numbers); writes the direction and its mixture density. */
+wl_status wl_guide_mixture_sample(wl_context ctx, const float *d_out, const float *d_u,
+                                  uint32_t n, float *d_dirs, float *d_pdf, void *stream);
+wl_status wl_guide_mixture_pdf(wl_context ctx, const float *d_out, const float *d_dirs,
+                               uint32_t n, float *d_pdf, void *stream);
+/* Mean loss and its gradient over n <= batch_size samples, no update. */
+wl_status wl_guide_gradient(wl_guide guide, const float *d_samples, uint32_t n,
+                            float *d_grad, float *d_loss, void *stream);
+/* `steps` Adam steps over consecutive mini-batches of the n samples. */
+wl_status wl_guide_train(wl_guide guide, const float *d_samples, uint32_t n,
+                         uint32_t steps, void *stream);
+/* Attaches (mode != 0) or detaches (guide NULL or mode 0) a guide. */
+wl_status wl_scene_set_guide(wl_scene scene, wl_guide guide, uint32_t mode);
 
 /* ---- programs, pipeline, SBT, launch: the OptiX program interface -------
  *

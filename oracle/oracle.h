@@ -194,6 +194,59 @@ void oracle_example_render(const oracle_pt *pt, const float *mesh_color,
                            const oracle_example_config *c, uint32_t row0,
                            uint32_t row1, float *rgba, int nthreads);
 
+/* ---- learned path guiding (no counterpart in the reference) -------------
+ * FP64 restatement of the guiding network that the library trains on the GPU
+ * (oracle_guide.c): a 64 -> 64 -> 64 -> 64 -> 64 -> 32 ReLU MLP whose 32
+ * outputs parameterise an 8-lobe von Mises-Fisher mixture over directions.
+ *
+ * Parameters (float, ORACLE_GUIDE_NPARAMS): for layer l = 0..4 the weight
+ * matrix W_l[out][in] row-major, then the bias b_l[out]; out = 64 for l < 4,
+ * 32 for l = 4; in = 64.
+ *
+ * Query (9 floats): position normalised to [0,1]^3, normal, outgoing
+ * direction.  Input encoding x[64]: for e < 36, frequency f = e / 6, axis
+ * d = (e % 6) / 2: sin (e even) or cos (e odd) of 2^f * pi * p[d];
+ * x[36..38] = normal, x[39..41] = outgoing direction, x[42..63] = 0.
+ *
+ * Output row r belongs to lobe 2 * ((r % 16) / 4) + r / 16, parameter r % 4:
+ * 0 = mixture logit a, 1 = raw concentration k (kappa = softplus(k)),
+ * 2 = theta, 3 = phi (mean direction (sin t cos p, sin t sin p, cos t)).
+ *
+ * Training sample (16 floats): query[9], sampled direction w[3], weight
+ * (= F / p of the sample, >= 0), then 3 unused.  Per-sample loss
+ * -weight * log q(w); the batch loss is the mean over the samples. */
+#define ORACLE_GUIDE_NPARAMS 18720
+#define ORACLE_GUIDE_NOUT 32
+#define ORACLE_GUIDE_LOBES 8
+
+/* Round-to-nearest-even float -> bfloat16 -> float. */
+float oracle_bf16_round(float x);
+void oracle_guide_encode(const float q[9], double x[64]);
+/* Network outputs.  emulate_bf16: round weights, layer inputs to bfloat16 as
+ * the GPU kernels do (accumulation stays FP64 here). */
+void oracle_guide_forward(const double *params, const float q[9], int emulate_bf16,
+                          double out[32]);
+/* Decoded mixture: weight pi, concentration kappa, mean direction mu[3]. */
+void oracle_guide_decode(const double out[32], double pi[8], double kappa[8],
+                         double mu[24]);
+double oracle_guide_pdf(const double out[32], const double w[3]);
+/* Sample: u0 picks the lobe by the cumulative weights, (u1, u2) the vMF
+ * direction (Jakob 2012 inversion) in the Duff et al. frame around mu. */
+void oracle_guide_sample(const double out[32], double u0, double u1, double u2,
+                         double w[3]);
+void oracle_guide_pdf_batch(const double out[32], const double *dirs, uint32_t n,
+                            double *pdf);
+void oracle_guide_sample_batch(const double out[32], const double *u, uint32_t n,
+                               double *dirs);
+/* Mean loss over n samples; grad (ORACLE_GUIDE_NPARAMS doubles) receives its
+ * gradient with respect to params.  emulate_bf16 rounds weights, layer
+ * inputs and back-propagated deltas to bfloat16 where the GPU kernels do. */
+double oracle_guide_loss_grad(const double *params, const float *samples, uint32_t n,
+                              int emulate_bf16, double *grad);
+/* One Adam step (Kingma & Ba 2015, bias-corrected), step = 1, 2, ... */
+void oracle_guide_adam(double *params, double *m, double *v, const double *grad,
+                       uint32_t step, double lr, double beta1, double beta2, double eps);
+
 #ifdef __cplusplus
 }
 #endif

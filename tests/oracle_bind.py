@@ -104,6 +104,20 @@ def lib():
         L.oracle_film_update.argtypes = [f32p, u8p, f32p, C.c_uint32, C.c_uint32]
         L.oracle_example_render.argtypes = [C.c_void_p, f32p, f32p, C.POINTER(ExampleConfig),
                                             C.c_uint32, C.c_uint32, f32p, C.c_int]
+        L.oracle_bf16_round.restype = C.c_float
+        L.oracle_bf16_round.argtypes = [C.c_float]
+        L.oracle_guide_encode.argtypes = [f32p, f64p]
+        L.oracle_guide_forward.argtypes = [f64p, f32p, C.c_int, f64p]
+        L.oracle_guide_decode.argtypes = [f64p, f64p, f64p, f64p]
+        L.oracle_guide_pdf.restype = C.c_double
+        L.oracle_guide_pdf.argtypes = [f64p, f64p]
+        L.oracle_guide_sample.argtypes = [f64p, C.c_double, C.c_double, C.c_double, f64p]
+        L.oracle_guide_pdf_batch.argtypes = [f64p, f64p, C.c_uint32, f64p]
+        L.oracle_guide_sample_batch.argtypes = [f64p, f64p, C.c_uint32, f64p]
+        L.oracle_guide_loss_grad.restype = C.c_double
+        L.oracle_guide_loss_grad.argtypes = [f64p, f32p, C.c_uint32, C.c_int, f64p]
+        L.oracle_guide_adam.argtypes = [f64p, f64p, f64p, f64p, C.c_uint32, C.c_double,
+                                        C.c_double, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -331,3 +345,67 @@ def film_update(acc_rgba, color_rgba, this_frame_rgb, frame):
     tf = np.ascontiguousarray(this_frame_rgb, np.float32)
     lib().oracle_film_update(_p(acc_rgba, f32p), _p(color_rgba, u8p), _p(tf, f32p),
                              npix, frame)
+
+
+# ---- learned guiding ----------------------------------------------------------
+GUIDE_PARAMS = 18720
+
+
+def bf16_round(x: float) -> float:
+    return float(lib().oracle_bf16_round(float(x)))
+
+
+def guide_encode(q):
+    q = np.ascontiguousarray(q, np.float32)
+    x = np.zeros(64, np.float64)
+    lib().oracle_guide_encode(_p(q, f32p), _p(x, f64p))
+    return x
+
+
+def guide_forward(params, queries, emulate_bf16=True):
+    P = np.ascontiguousarray(params, np.float64)
+    Q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+    out = np.zeros((Q.shape[0], 32), np.float64)
+    for i in range(Q.shape[0]):
+        lib().oracle_guide_forward(_p(P, f64p), _p(Q[i], f32p), int(emulate_bf16),
+                                   _p(out[i], f64p))
+    return out
+
+
+def guide_decode(out):
+    o = np.ascontiguousarray(out, np.float64)
+    pi, kap, mu = np.zeros(8), np.zeros(8), np.zeros(24)
+    lib().oracle_guide_decode(_p(o, f64p), _p(pi, f64p), _p(kap, f64p), _p(mu, f64p))
+    return pi, kap, mu.reshape(8, 3)
+
+
+def guide_pdf(out, dirs):
+    o = np.ascontiguousarray(out, np.float64)
+    D = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    pdf = np.zeros(D.shape[0])
+    lib().oracle_guide_pdf_batch(_p(o, f64p), _p(D, f64p), D.shape[0], _p(pdf, f64p))
+    return pdf
+
+
+def guide_sample(out, u):
+    o = np.ascontiguousarray(out, np.float64)
+    U = np.ascontiguousarray(u, np.float64).reshape(-1, 3)
+    w = np.zeros((U.shape[0], 3))
+    lib().oracle_guide_sample_batch(_p(o, f64p), _p(U, f64p), U.shape[0], _p(w, f64p))
+    return w
+
+
+def guide_loss_grad(params, samples, emulate_bf16=False):
+    P = np.ascontiguousarray(params, np.float64)
+    S = np.ascontiguousarray(samples, np.float32).reshape(-1, 16)
+    g = np.zeros(GUIDE_PARAMS, np.float64)
+    loss = lib().oracle_guide_loss_grad(_p(P, f64p), _p(S, f32p), S.shape[0],
+                                        int(emulate_bf16), _p(g, f64p))
+    return loss, g
+
+
+def guide_adam(params, m, v, grad, step, lr, beta1, beta2, eps):
+    """In place on float64 arrays."""
+    lib().oracle_guide_adam(_p(params, f64p), _p(m, f64p), _p(v, f64p),
+                            _p(np.ascontiguousarray(grad, np.float64), f64p),
+                            step, lr, beta1, beta2, eps)

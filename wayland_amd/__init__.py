@@ -261,6 +261,110 @@ def float_to_u8(src):
     return dst
 
 
+GUIDE_PARAMS = 18720
+GUIDE_SAMPLE = 1
+GUIDE_TRAIN = 2
+
+
+class Guide:
+    """Learned guiding network: 64 -> 4 x 64 -> 32 MLP (BF16 MFMA, FP32
+    master weights) giving an 8-lobe von Mises-Fisher mixture.  No
+    counterpart in the reference."""
+
+    def __init__(self, ctx: Context, **cfg):
+        c = _lib.GuideConfig()
+        _lib.call("wl_guide_default_config", C.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown guide option {k}")
+            setattr(c, k, v)
+        self.ctx, self.cfg = ctx, c
+        h = C.c_void_p()
+        _lib.call("wl_guide_create", ctx.h, C.byref(c), C.byref(h))
+        self.h = h
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(GUIDE_PARAMS, np.float32)
+        _lib.call("wl_guide_get_params", self.h,
+                  out.ctypes.data_as(C.POINTER(C.c_float)), GUIDE_PARAMS)
+        return out
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != GUIDE_PARAMS:
+            raise ValueError("wrong parameter count")
+        _lib.call("wl_guide_set_params", self.h,
+                  p.ctypes.data_as(C.POINTER(C.c_float)), GUIDE_PARAMS)
+
+    def set_mix(self, alpha: float):
+        _lib.call("wl_guide_set_mix", self.h, float(alpha))
+
+    def info(self) -> dict:
+        i = _lib.GuideInfo()
+        _lib.call("wl_guide_get_info", self.h, C.byref(i))
+        return {"steps": i.steps, "last_samples": i.last_samples}
+
+    def infer(self, queries, out=None):
+        torch = _torch()
+        q = queries.contiguous()
+        n = q.shape[0]
+        if out is None:
+            out = torch.empty((n, 32), dtype=torch.float32, device=q.device)
+        _lib.call("wl_guide_infer", self.h, C.c_void_p(q.data_ptr()), n,
+                  C.c_void_p(out.data_ptr()), _stream_ptr(q.device))
+        return out
+
+    def gradient(self, samples):
+        torch = _torch()
+        smp = samples.contiguous()
+        grad = torch.empty(GUIDE_PARAMS, dtype=torch.float32, device=smp.device)
+        loss = torch.empty(1, dtype=torch.float32, device=smp.device)
+        _lib.call("wl_guide_gradient", self.h, C.c_void_p(smp.data_ptr()), smp.shape[0],
+                  C.c_void_p(grad.data_ptr()), C.c_void_p(loss.data_ptr()),
+                  _stream_ptr(smp.device))
+        return grad, loss
+
+    def train(self, samples, steps: int):
+        smp = samples.contiguous()
+        _lib.call("wl_guide_train", self.h, C.c_void_p(smp.data_ptr()), smp.shape[0],
+                  int(steps), _stream_ptr(smp.device))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_guide_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mixture_sample(ctx: Context, outs, u):
+    """Directions and mixture densities for given network outputs and
+    uniforms (3 per query)."""
+    torch = _torch()
+    o, uu = outs.contiguous(), u.contiguous()
+    n = o.shape[0]
+    dirs = torch.empty((n, 3), dtype=torch.float32, device=o.device)
+    pdf = torch.empty(n, dtype=torch.float32, device=o.device)
+    _lib.call("wl_guide_mixture_sample", ctx.h, C.c_void_p(o.data_ptr()),
+              C.c_void_p(uu.data_ptr()), n, C.c_void_p(dirs.data_ptr()),
+              C.c_void_p(pdf.data_ptr()), _stream_ptr(o.device))
+    return dirs, pdf
+
+
+def mixture_pdf(ctx: Context, outs, dirs):
+    torch = _torch()
+    o, d = outs.contiguous(), dirs.contiguous()
+    n = o.shape[0]
+    pdf = torch.empty(n, dtype=torch.float32, device=o.device)
+    _lib.call("wl_guide_mixture_pdf", ctx.h, C.c_void_p(o.data_ptr()),
+              C.c_void_p(d.data_ptr()), n, C.c_void_p(pdf.data_ptr()), _stream_ptr(o.device))
+    return pdf
+
+
 class PathTracer:
     """The PathTracing program on a scene (reference:
     src/Frontend/Programs/PathTracing/*)."""
@@ -334,6 +438,13 @@ class PathTracer:
         _lib.call("wl_film_resolve", self.ctx.h, C.c_void_p(film.data_ptr()), npix,
                   nsamples, C.c_void_p(out.data_ptr()), _stream_ptr(film.device))
         return out
+
+    def set_guide(self, guide, mode=GUIDE_SAMPLE | GUIDE_TRAIN):
+        """Attach a Guide (mode GUIDE_SAMPLE, optionally | GUIDE_TRAIN) or
+        detach (guide None).  The guide must outlive the attachment."""
+        _lib.call("wl_scene_set_guide", self.h, guide.h if guide is not None else None,
+                  int(mode) if guide is not None else 0)
+        self.guide = guide
 
     def close(self):
         if getattr(self, "h", None):

@@ -67,6 +67,18 @@ class PTStats(C.Structure):
                 ("pad_", C.c_uint32), ("seconds", C.c_double)]
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("batch_size", C.c_uint32),
+                ("steps_per_batch", C.c_uint32), ("max_samples", C.c_uint32),
+                ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("epsilon", C.c_float), ("mix_alpha", C.c_float), ("max_weight", C.c_float),
+                ("pad_", C.c_uint32 * 2)]
+
+
+class GuideInfo(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("last_samples", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class ProgramGroup(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("entry", (C.c_char * 64) * 3)]
 
@@ -144,6 +156,19 @@ SIGNATURES = {
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
     "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),
     "wl_film_resolve": (C.c_int, [_VP, _VP, _U32, _U32, _VP, _VP]),
+    "wl_guide_default_config": (C.c_int, [C.POINTER(GuideConfig)]),
+    "wl_guide_create": (C.c_int, [_VP, C.POINTER(GuideConfig), C.POINTER(_VP)]),
+    "wl_guide_destroy": (C.c_int, [_VP]),
+    "wl_guide_set_params": (C.c_int, [_VP, C.POINTER(C.c_float), _U32]),
+    "wl_guide_get_params": (C.c_int, [_VP, C.POINTER(C.c_float), _U32]),
+    "wl_guide_set_mix": (C.c_int, [_VP, C.c_float]),
+    "wl_guide_get_info": (C.c_int, [_VP, C.POINTER(GuideInfo)]),
+    "wl_guide_infer": (C.c_int, [_VP, _VP, _U32, _VP, _VP]),
+    "wl_guide_mixture_sample": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP, _VP]),
+    "wl_guide_mixture_pdf": (C.c_int, [_VP, _VP, _VP, _U32, _VP, _VP]),
+    "wl_guide_gradient": (C.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
+    "wl_guide_train": (C.c_int, [_VP, _VP, _U32, _U32, _VP]),
+    "wl_scene_set_guide": (C.c_int, [_VP, _VP, _U32]),
     "wl_identify_programs": (C.c_int, [C.c_char_p, C.c_size_t, _VP, _U32, C.POINTER(_U32)]),
     "wl_pipeline_create": (C.c_int, [_VP, _VP, _U32, _U32, C.POINTER(_VP), C.POINTER(_U32)]),
     "wl_pipeline_destroy": (C.c_int, [_VP]),

@@ -27,7 +27,9 @@ SOURCES = [
     "wl_api.cpp",
     "wl_pt_api.cpp",
     "wl_program.cpp",
+    "wl_guide_api.cpp",
     "traverse.hip",
+    "guide.hip",
     "pathtrace.hip",
     "example_pt.hip",
 ]

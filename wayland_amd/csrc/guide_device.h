@@ -1,0 +1,210 @@
+// guide_device.h — the learned guiding distribution on gfx950: input
+// encoding, the 64-64-64-64-64-32 MLP on BF16 MFMA tiles, and the 8-lobe von
+// Mises-Fisher mixture its outputs parameterise.  Shared by the guide
+// kernels (guide.hip) and the path tracer's shade kernel (pathtrace.hip).
+//
+// Parameters (FP32 master copy, 18720 floats): layer l = 0..4 holds
+// W_l[out][in] row-major then b_l[out], out = 64 (l < 4) or 32 (l = 4),
+// in = 64.  Output row r is lobe 2 * ((r % 16) / 4) + r / 16, parameter r % 4:
+// 0 logit, 1 raw concentration (kappa = softplus), 2 theta, 3 phi.
+//
+// MFMA tiling (v_mfma_f32_16x16x32_bf16, one wave, 16 queries per tile).
+// Lane l, lane group g = l / 16, element j:
+//   A (16 x 32):  row l % 16,    k = 8 g + j       (8 bf16 per lane)
+//   B (32 x 16):  column l % 16, k = 8 g + j       (8 bf16 per lane)
+//   D (16 x 16):  column l % 16, row 4 g + j       (4 floats per lane)
+// Layers run transposed: H_out^T = W^T-tile x H_in^T, so a lane always works
+// on query l % 16 and the result tile t of a layer leaves lane group g with
+// features 16 t + 4 g + i.  The next layer's B operand takes, in k-block kb,
+// element j, the feature phi(kb, g, j) = 16 (2 kb + j / 4) + 4 g + j % 4 —
+// exactly the values the lane already holds — and the packed weights use the
+// same k order, so activations never cross lanes between layers.
+#pragma once
+#include "guide_layout.h"
+#include "shading_device.h"
+
+namespace wl
+{
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+__device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ---- input encoding ------------------------------------------------------------
+// Query q[9]: position in [0,1]^3, normal, outgoing direction.  Feature e:
+// e < 36: sin (e even) / cos (e odd) of 2^(e/6) * pi * p[(e % 6) / 2];
+// 36..38 normal, 39..41 outgoing direction, 42..63 zero.
+// The lane's features come in pairs (e, e + 1) sharing one argument.  Which
+// coordinate a lane reads depends on its lane group, so q lives in LDS (a
+// private array indexed per lane would go to scratch memory).
+__device__ __forceinline__ void guide_encode(const float *q, int g, bf16x8 x[2])
+{
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; j += 2)
+        {
+            const int e = guide_phi(kb, g, j); // even; e + 1 is the cosine
+            float v0, v1;
+            if (e < 36)
+            {
+                const int f = e / 6, d = (e % 6) / 2;
+                // sin / cos of pi * 2^f * p; sincospi reduces its argument exactly
+                sincospif(ldexpf(q[d], f), &v0, &v1);
+            }
+            else if (e < 42)
+            {
+                v0 = q[e - 33]; // normal then outgoing direction
+                v1 = q[e - 32];
+            }
+            else
+            {
+                v0 = 0.f;
+                v1 = 0.f;
+            }
+            x[kb][j] = (__bf16)v0;
+            x[kb][j + 1] = (__bf16)v1;
+        }
+}
+
+// ---- forward pass of one 16-query tile -------------------------------------
+// A: packed forward operands (LDS or global), bias: 288 FP32 biases in layer
+// order.  x: layer-0 input on entry.  keep (optional): the B operand that
+// entered each layer.  out: output tiles t = 0, 1 (rows 16 t + 4 g + i).
+template <bool KEEP>
+__device__ __forceinline__ void guide_forward(const bf16x8 *A, const float *bias, int lane,
+                                              bf16x8 x[2], floatx4 out[2], bf16x8 keep[5][2])
+{
+    const int g = lane >> 4;
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+    {
+        floatx4 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+        {
+            const float4 b = *reinterpret_cast<const float4 *>(bias + 64 * l + 16 * t + 4 * g);
+            acc[t] = floatx4{ b.x, b.y, b.z, b.w };
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb)
+                acc[t] = mfma16(A[guide_fwd_vec(l, t, kb) + lane], x[kb], acc[t]);
+        }
+        if (KEEP)
+        {
+            keep[l][0] = x[0];
+            keep[l][1] = x[1];
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                x[kb][j] = (__bf16)fmaxf(acc[2 * kb + j / 4][j % 4], 0.0f);
+    }
+    if (KEEP)
+    {
+        keep[4][0] = x[0];
+        keep[4][1] = x[1];
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+    {
+        const float4 b = *reinterpret_cast<const float4 *>(bias + 256 + 16 * t + 4 * g);
+        out[t] = floatx4{ b.x, b.y, b.z, b.w };
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+            out[t] = mfma16(A[guide_fwd_vec(4, t, kb) + lane], x[kb], out[t]);
+    }
+}
+
+// ---- von Mises-Fisher mixture ------------------------------------------------
+__device__ __forceinline__ float guide_softplus(float k) { return k > 15.f ? k : log1pf(expf(k)); }
+
+// log of kappa / (2 pi (1 - e^{-2 kappa}))
+__device__ __forceinline__ float vmf_log_norm(float kappa)
+{
+    if (kappa < 1e-4f)
+        return logf((1.0f + kappa) * 0.0795774715f);
+    return logf(kappa) - 1.83787706641f - logf(-expm1f(-2.0f * kappa));
+}
+
+struct GuideMix
+{
+    float pi[kGuideLobes];
+    float kappa[kGuideLobes];
+    float lnorm[kGuideLobes];
+    V3 mu[kGuideLobes];
+};
+
+// o: the 32 outputs in row order.
+__device__ __forceinline__ void guide_decode(const float o[kGuideOut], GuideMix &m)
+{
+    float amax = -3.0e38f;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+        amax = fmaxf(amax, o[guide_row(j, 0)]);
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+    {
+        m.pi[j] = expf(o[guide_row(j, 0)] - amax);
+        s += m.pi[j];
+    }
+    const float inv = 1.0f / s;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+    {
+        m.pi[j] *= inv;
+        m.kappa[j] = guide_softplus(o[guide_row(j, 1)]);
+        m.lnorm[j] = vmf_log_norm(m.kappa[j]);
+        float st, ct, sp, cp;
+        sincosf(o[guide_row(j, 2)], &st, &ct);
+        sincosf(o[guide_row(j, 3)], &sp, &cp);
+        m.mu[j] = mk(st * cp, st * sp, ct);
+    }
+}
+
+__device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w)
+{
+    float q = 0.f;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+        q += m.pi[j] * expf(m.lnorm[j] + m.kappa[j] * (dot(m.mu[j], w) - 1.0f));
+    return q;
+}
+
+// u0 picks the lobe, (u1, u2) the direction (Jakob 2012 inversion).
+__device__ __forceinline__ V3 guide_sample(const GuideMix &m, float u0, float u1, float u2)
+{
+    float c = 0.f, kap = m.kappa[kGuideLobes - 1];
+    V3 mu = m.mu[kGuideLobes - 1];
+    bool found = false;
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+    {
+        c += m.pi[j];
+        if (!found && u0 < c)
+        {
+            found = true;
+            kap = m.kappa[j];
+            mu = m.mu[j];
+        }
+    }
+    // w_z = 1 + t, t = log(u + (1 - u) e^{-2 kappa}) / kappa, written with
+    // log1p / expm1 so small kappa does not cancel; 1 - w_z^2 = -t (2 + t)
+    float t = kap < 1e-4f ? 2.0f * u1 - 2.0f
+                          : log1pf((1.0f - u1) * expm1f(-2.0f * kap)) / kap;
+    t = clampf(t, -2.0f, 0.0f);
+    const float wz = 1.0f + t;
+    const float r = sqrtf(fmaxf(0.0f, -t * (2.0f + t)));
+    float sp, cp;
+    sincosf(6.28318530718f * u2, &sp, &cp);
+    V3 T, Bv;
+    onb(mu, T, Bv);
+    return T * (r * cp) + Bv * (r * sp) + mu * wz;
+}
+
+} // namespace wl

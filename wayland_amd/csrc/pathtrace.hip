@@ -21,6 +21,7 @@
 // bounce at most), so per-sample radiance matches a serial evaluation.
 // Every kernel loops over a device-side queue count with a fixed grid, so the
 // host enqueues a whole batch without synchronising.
+#include "guide_device.h"
 #include "shading_device.h"
 #include "traverse_persistent.h"
 #include "wl_pt.h"
@@ -73,6 +74,8 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
         B.rad[p] = f4(mk(0, 0, 0), 0.0f);
         B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
         B.qpath0[p] = p;
+        if (P.gmode & 2u)
+            B.gacc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.counters[0] = npaths;
@@ -104,7 +107,93 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
         });
 }
 
+// ---- guide inference for the active queue ------------------------------------
+// Query of a hit: position normalised to the scene box, geometric normal
+// facing the incoming ray, outgoing direction.  The shade kernel builds the
+// same query from the same values for its training records.
+__device__ __forceinline__ void guide_query(const PtParams &P, V3 hitPos, V3 Ng, V3 rayDir,
+                                            float q[9])
+{
+    q[0] = (hitPos.x - P.gbox_lo[0]) * P.gbox_inv[0];
+    q[1] = (hitPos.y - P.gbox_lo[1]) * P.gbox_inv[1];
+    q[2] = (hitPos.z - P.gbox_lo[2]) * P.gbox_inv[2];
+    q[3] = Ng.x;
+    q[4] = Ng.y;
+    q[5] = Ng.z;
+    q[6] = -rayDir.x;
+    q[7] = -rayDir.y;
+    q[8] = -rayDir.z;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_guide_infer_pt(PtParams P, PtScene S,
+                                                             PtBuffers B, int parity)
+{
+    __shared__ bf16x8 sA[kGuideFwdVecs];
+    __shared__ float sb[kGuideBiases];
+    __shared__ float sq[9 * kPtBlock];
+    float *q = sq + 9 * threadIdx.x;
+    const bf16x8 *fwd = reinterpret_cast<const bf16x8 *>(P.gfwd);
+    for (int k = threadIdx.x; k < kGuideFwdVecs; k += kPtBlock)
+        sA[k] = fwd[k];
+    for (int k = threadIdx.x; k < kGuideBiases; k += kPtBlock)
+        sb[k] = k < 256 ? P.gparams[guide_b_off(k / 64) + k % 64]
+                        : P.gparams[guide_b_off(4) + (k - 256)];
+    __syncthreads();
+    const uint32_t count = B.counters[parity];
+    const wl_ray *rays = parity ? B.rays1 : B.rays0;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const uint32_t tiles = (count + 15) / 16;
+    const uint32_t nwaves = gridDim.x * (kPtBlock / 64);
+    for (uint32_t tile = blockIdx.x * (kPtBlock / 64) + threadIdx.x / 64; tile < tiles;
+         tile += nwaves)
+    {
+        const uint32_t i = tile * 16 + (lane & 15);
+        bool valid = i < count;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            q[k] = 0.f;
+        if (valid)
+        {
+            const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
+            const uint32_t prim = __float_as_uint(hit.y);
+            valid = prim != WL_INVALID;
+            if (valid)
+            {
+                const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+                const float4 r0 = rp[0], r1 = rp[1];
+                const V3 o = mk(r0.x, r0.y, r0.z), d = mk(r1.x, r1.y, r1.z);
+                const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
+                const float4 w0 = sr[0], w1 = sr[1], w2 = sr[2];
+                const V3 va = mk(w0.x, w0.y, w0.z), vb = mk(w0.w, w1.x, w1.y),
+                         vc = mk(w1.z, w1.w, w2.x);
+                V3 Ng = normalize(cross(vb - va, vc - va));
+                if (dot(Ng, d) > 0)
+                    Ng = -Ng;
+                guide_query(P, o + d * hit.x, Ng, d, q);
+            }
+        }
+        bf16x8 x[2];
+        guide_encode(q, g, x);
+        floatx4 o[2];
+        guide_forward<false>(sA, sb, lane, x, o, nullptr);
+        if (valid)
+        {
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                *reinterpret_cast<float4 *>(B.gout + 32 * (size_t)i + 16 * t + 4 * g) =
+                    make_float4(o[t][0], o[t][1], o[t][2], o[t][3]);
+        }
+    }
+}
+
 // ---- shade ------------------------------------------------------------------
+// GUIDED: one-sample MIS of the learned guide and the BSDF at vertices whose
+// material has trans = 0 and clearcoat = 0.  There the reference's pdf equals
+// the density its sampler draws from wherever the BSDF is non-zero, so
+// mixing in the guide keeps the estimator's expectation unchanged.  The mixed
+// density replaces the BSDF pdf everywhere the reference uses it (the path
+// weight, lastPdf for emitter-hit MIS, the NEE MIS weight).
+template <bool GUIDED>
 __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuffers B,
                                                     int parity)
 {
@@ -115,7 +204,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
     for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
     {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, shadow = false;
+        bool cont = false, shadow = false, shadow_gacc = false;
         uint32_t path = 0;
         wl_ray next_ray{}, shadow_ray{};
         V3 contrib = mk(0, 0, 0);
@@ -136,6 +225,14 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
             V3 radiance = v3of(s_rad);
             const uint32_t prim = __float_as_uint(hit.y);
             bool done = false;
+            // guiding: training record state of the path
+            const bool gtrain = GUIDED && (P.gmode & 2u);
+            float4 gacc = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gtrain)
+                gacc = B.gacc[path];
+            const bool grec_done = gacc.w != 0.f; // first guided vertex recorded
+            bool grec_now = false;
+            const V3 rad_in = radiance;
             if (prim == WL_INVALID)
             {
                 // __miss__radiance: radiance += throughput * bg
@@ -202,9 +299,46 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                         if (dot(Ns, rayDir) > 0)
                             Ns = -Ns;
                     }
-                    V3 nd = sample_disney(mat, Ns, -rayDir, seed);
-                    nd = normalize(nd);
-                    const float pdf = pdf_disney(mat, Ns, -rayDir, nd);
+                    const bool guided =
+                        GUIDED && mat.trans == 0.0f && mat.clearcoat == 0.0f && (P.gmode & 1u);
+                    GuideMix gm;
+                    bool took_guide = false;
+                    V3 nd;
+                    if (GUIDED && guided)
+                    {
+                        float o[kGuideOut];
+                        const float4 *gp = reinterpret_cast<const float4 *>(B.gout + 32 * (size_t)i);
+#pragma unroll
+                        for (int k = 0; k < kGuideOut / 4; ++k)
+                        {
+                            const float4 v = gp[k];
+                            o[4 * k] = v.x;
+                            o[4 * k + 1] = v.y;
+                            o[4 * k + 2] = v.z;
+                            o[4 * k + 3] = v.w;
+                        }
+                        guide_decode(o, gm);
+                        if (rnd(seed) < P.galpha)
+                        {
+                            const float u0 = rnd(seed);
+                            const float u1 = rnd(seed);
+                            const float u2 = rnd(seed);
+                            nd = guide_sample(gm, u0, u1, u2);
+                            took_guide = true;
+                        }
+                    }
+                    if (!took_guide)
+                    {
+                        nd = sample_disney(mat, Ns, -rayDir, seed);
+                        nd = normalize(nd);
+                    }
+                    float pdf = pdf_disney(mat, Ns, -rayDir, nd);
+                    if (GUIDED && guided && !(pdf < 0.0f || __builtin_isnan(pdf)))
+                    {
+                        pdf = P.galpha * guide_pdf(gm, nd) + (1.0f - P.galpha) * pdf;
+                        if (!(pdf > 0.0f))
+                            pdf = -1.0f; // a zero mixed density ends the path
+                    }
                     if (pdf < 0.0f || __builtin_isnan(pdf))
                     {
                         done = true;
@@ -217,6 +351,21 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                         bsdf = clamp3(bsdf, 0.f, 1e30f);
                         lastTraceTerm = bsdf * fabsf(cosWeight) / pdf;
                         const V3 lastRayDir = rayDir;
+                        if (GUIDED && guided && gtrain && !grec_done)
+                        {
+                            // training record: query, sampled direction and
+                            // the throughput reaching this vertex
+                            float q[9];
+                            guide_query(P, hitPos, Ng, rayDir, q);
+                            float4 *r = reinterpret_cast<float4 *>(B.grec + 16 * (size_t)path);
+                            r[0] = make_float4(q[0], q[1], q[2], q[3]);
+                            r[1] = make_float4(q[4], q[5], q[6], q[7]);
+                            r[2] = make_float4(q[8], nd.x, nd.y, nd.z);
+                            r[3] = make_float4(0.0f, (throughput.x + throughput.y + throughput.z) *
+                                                         (1.0f / 3.0f),
+                                               0.0f, 0.0f);
+                            grec_now = true;
+                        }
                         rayPos = hitPos;
                         rayDir = nd;
                         lastPdf = pdf;
@@ -270,6 +419,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                                     uptPdf = pdf_disney(mat, Ns, -lastRayDir, visRay);
                                     if (uptPdf < 0 || __builtin_isnan(uptPdf))
                                         uptPdf = 0.f;
+                                    if (GUIDED && guided)
+                                        uptPdf = P.galpha * guide_pdf(gm, visRay) +
+                                                 (1.0f - P.galpha) * uptPdf;
                                 }
                                 V3 bv = eval_disney(mat, Ns, Ng, -lastRayDir, visRay, mk(1, 1, 1));
                                 bv = clamp3(bv, 0.f, 1e30f);
@@ -296,6 +448,21 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                 }
             }
             B.rad[path] = f4(radiance, 0.0f);
+            if (gtrain)
+            {
+                // radiance reaching the path after its first guided vertex
+                if (grec_done)
+                {
+                    gacc.x += radiance.x - rad_in.x;
+                    gacc.y += radiance.y - rad_in.y;
+                    gacc.z += radiance.z - rad_in.z;
+                }
+                if (grec_now)
+                    gacc.w = 1.0f;
+                if (grec_done || grec_now)
+                    B.gacc[path] = gacc;
+                shadow_gacc = grec_done;
+            }
             if (!done)
             {
                 cont = true;
@@ -319,7 +486,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
         {
             B.spath[sslot] = path;
             B.srays[sslot] = shadow_ray;
-            B.scontrib[sslot] = f4(contrib, 0.0f);
+            B.scontrib[sslot] = f4(contrib, shadow_gacc ? 1.0f : 0.0f);
         }
     }
 }
@@ -350,7 +517,54 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
             rr.y = rr.y + c.y;
             rr.z = rr.z + c.z;
             B.rad[path] = rr;
+            if (c.w != 0.0f) // the path records guide training radiance
+            {
+                float4 ga = B.gacc[path];
+                ga.x += c.x;
+                ga.y += c.y;
+                ga.z += c.z;
+                B.gacc[path] = ga;
+            }
         });
+}
+
+// ---- guide training samples -----------------------------------------------------
+// A recorded vertex's weight is F/p of its sampled direction: the radiance
+// the path gathered through that direction divided by the throughput that
+// reached the vertex (channel means).  Zero-weight records carry no gradient
+// and are dropped.
+__global__ __launch_bounds__(kPtBlock) void k_guide_collect(PtParams P, PtBuffers B)
+{
+    const uint32_t npaths = P.npix * P.nsamples;
+    const uint32_t nwaves_all = (npaths + 63) / 64;
+    for (uint32_t base = blockIdx.x * kPtBlock; base < nwaves_all * 64u;
+         base += gridDim.x * kPtBlock)
+    {
+        const uint32_t p = base + threadIdx.x;
+        bool keep = false;
+        float w = 0.f;
+        if (p < npaths)
+        {
+            const float4 ga = B.gacc[p];
+            if (ga.w != 0.0f)
+            {
+                const float thr = B.grec[16 * (size_t)p + 13];
+                w = (ga.x + ga.y + ga.z) * (1.0f / 3.0f) / fmaxf(thr, 1e-30f);
+                keep = w > 0.0f; // false for NaN
+                w = fminf(w, B.gwmax);
+            }
+        }
+        const uint32_t slot = wave_append(B.gcount, keep);
+        if (keep && slot < B.gtrain_cap)
+        {
+            const float4 *src = reinterpret_cast<const float4 *>(B.grec + 16 * (size_t)p);
+            float4 *dst = reinterpret_cast<float4 *>(B.gtrain + 16 * (size_t)slot);
+            dst[0] = src[0];
+            dst[1] = src[1];
+            dst[2] = src[2];
+            dst[3] = make_float4(w, 0.0f, 0.0f, 0.0f);
+        }
+    }
 }
 
 // ---- film -----------------------------------------------------------------
@@ -464,6 +678,9 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     const uint32_t g_trav =
         std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (uint32_t)(shallow ? occ16 : occ32));
     const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
+    // guide inference: 64 queries per workgroup pass, up to 4 workgroups per CU
+    const uint32_t g_guide =
+        std::max<uint32_t>(1, std::min<uint32_t>((npaths + 63) / 64, (uint32_t)num_cus * 4));
     hipLaunchKernelGGL(k_raygen, dim3(g_gen), dim3(kPtBlock), 0, st, P, B);
     for (uint32_t b = 0; b < P.max_depth; ++b)
     {
@@ -473,13 +690,31 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
         else
             hipLaunchKernelGGL(k_extend<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
                                S, B, parity);
-        hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B, parity);
+        if (P.gmode)
+        {
+            hipLaunchKernelGGL(k_guide_infer_pt, dim3(g_guide), dim3(kPtBlock), 0, st, P, S, B,
+                               parity);
+            hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B,
+                               parity);
+        }
+        else
+            hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B,
+                               parity);
         if (shallow)
             hipLaunchKernelGGL(k_shadow<16>, dim3(g_trav), dim3(kPtBlock), 0, st, S, B);
         else
             hipLaunchKernelGGL(k_shadow<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
                                S, B);
     }
+    return hipGetLastError();
+}
+
+hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus, hipStream_t st)
+{
+    const uint32_t npaths = P.npix * P.nsamples;
+    const uint32_t g =
+        std::max<uint32_t>(1, std::min<uint32_t>((npaths + kPtBlock - 1) / kPtBlock, (uint32_t)num_cus * 8));
+    hipLaunchKernelGGL(k_guide_collect, dim3(g), dim3(kPtBlock), 0, st, P, B);
     return hipGetLastError();
 }
 

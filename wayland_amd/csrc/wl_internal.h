@@ -9,6 +9,7 @@
 
 #include "../../include/wayland_hip.h"
 #include "bvh8.h"
+#include "wl_guide.h"
 
 // Reference counted: the caller's handle plus one reference per accel, scene
 // and pipeline made from it, so children may be released in any order.
@@ -63,6 +64,22 @@ struct wl_scene_s
     void *d_batch = nullptr;                  // one allocation, carved up
     uint32_t *d_counters = nullptr;
     unsigned long long *d_stats = nullptr;
+    // learned guiding
+    wl_guide guide = nullptr;
+    uint32_t guide_mode = 0;
+    float box_lo[3] = { 0, 0, 0 }, box_inv[3] = { 1, 1, 1 }; // scene bounds
+    uint32_t gcapacity = 0;                   // paths covered by d_gbatch
+    void *d_gbatch = nullptr;                 // gout, gacc, grec
+};
+
+struct wl_guide_s
+{
+    wl_context ctx = nullptr;
+    wl_guide_config cfg{};
+    wl::GuideDev dev{};
+    void *d_alloc = nullptr;
+    uint64_t steps = 0;
+    hipStream_t last_stream = nullptr;
 };
 
 namespace wl
@@ -82,6 +99,9 @@ hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
                                 hipStream_t s);
 hipError_t launch_trace_any(const wl_accel_s *a, const wl_ray *rays,
                             uint32_t n, uint32_t *bits, hipStream_t s);
+// Adam steps of a guide over samples[0, min(*n_dev, n_cap)) (wl_guide_api.cpp)
+wl_status guide_train_device(wl_guide g, const float *samples, const uint32_t *n_dev,
+                             uint32_t n_cap, uint32_t steps, hipStream_t s);
 } // namespace wl
 
 #define WL_HIP_CHECK(expr)                                                   \

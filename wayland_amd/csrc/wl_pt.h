@@ -55,6 +55,13 @@ struct PtParams
     uint32_t strategy, num_lights;
     float bg[3];
     wl_pinhole_camera cam;
+    // learned guiding (gmode 0 = off): bit 0 samples the guide at vertices
+    // whose material is guidable, bit 1 also records training samples
+    uint32_t gmode;
+    float galpha;              // probability of sampling the guide
+    const void *gfwd;          // packed BF16 forward operands
+    const float *gparams;      // FP32 master parameters (biases)
+    float gbox_lo[3], gbox_inv[3]; // position normalisation to [0,1]^3
 };
 
 // Path state (structure of float4 arrays, one entry per path) and queues.
@@ -73,10 +80,23 @@ struct PtBuffers
     float4 *scontrib;
     uint32_t *counters;       // [0], [1] active queues, [2] shadow queue
     unsigned long long *stats; // [0] extension rays, [1] shadow rays
+    // learned guiding (only with PtParams::gmode != 0)
+    float *gout;      // network outputs, 32 floats per queue slot
+    float4 *gacc;     // per path: radiance added after its first guided
+                      // vertex; w = 1 once that vertex is recorded
+    float *grec;      // per path: training record of that vertex (16 floats)
+    float *gtrain;    // training samples (16 floats each)
+    uint32_t *gcount; // number of training samples
+    uint32_t gtrain_cap;
+    float gwmax;      // training weights are clamped to this
 };
 
 hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                         int num_cus, hipStream_t st);
+// Moves the finished batch's training records with a positive weight into
+// the training buffer.
+hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus,
+                            hipStream_t st);
 hipError_t pt_accumulate(const PtParams &P, const PtBuffers &B, float *film,
                          int num_cus, hipStream_t st);
 hipError_t pt_store_samples(const PtParams &P, const PtBuffers &B, float *rgb,

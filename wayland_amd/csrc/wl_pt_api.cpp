@@ -41,6 +41,20 @@ wl::PtBuffers carve(wl_scene s)
     B.scontrib = (float4 *)take(16 * cap);
     B.counters = s->d_counters;
     B.stats = s->d_stats;
+    if (s->guide && s->guide_mode && s->d_gbatch)
+    {
+        char *g = (char *)s->d_gbatch;
+        B.gout = (float *)g;
+        g += (128 * cap + 255) & ~(size_t)255;
+        B.gacc = (float4 *)g;
+        g += (16 * cap + 255) & ~(size_t)255;
+        B.grec = (float *)g;
+        wl::GuideDev &D = s->guide->dev;
+        B.gtrain = D.train;
+        B.gcount = D.count;
+        B.gtrain_cap = D.train_cap;
+        B.gwmax = s->guide->cfg.max_weight;
+    }
     return B;
 }
 
@@ -50,8 +64,27 @@ size_t batch_bytes(size_t cap)
     return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 3 + r(16 * cap) * 2;
 }
 
+size_t guide_batch_bytes(size_t cap)
+{
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return r(128 * cap) + r(16 * cap) + r(64 * cap);
+}
+
 wl_status ensure_capacity(wl_scene s, uint32_t paths)
 {
+    if (s->guide && s->guide_mode && s->gcapacity < std::max(paths, s->capacity))
+    {
+        const uint32_t gcap = std::max(paths, s->capacity);
+        if (s->d_gbatch)
+        {
+            WL_HIP_CHECK(hipDeviceSynchronize());
+            WL_HIP_CHECK(hipFree(s->d_gbatch));
+            s->d_gbatch = nullptr;
+            s->gcapacity = 0;
+        }
+        WL_HIP_CHECK(hipMalloc(&s->d_gbatch, guide_batch_bytes(gcap)));
+        s->gcapacity = gcap;
+    }
     if (paths <= s->capacity)
         return WL_SUCCESS;
     if (s->d_batch)
@@ -63,6 +96,22 @@ wl_status ensure_capacity(wl_scene s, uint32_t paths)
     }
     WL_HIP_CHECK(hipMalloc(&s->d_batch, batch_bytes(paths)));
     s->capacity = paths;
+    return WL_SUCCESS;
+}
+
+// Guided batches: after the bounces, move the training records into the
+// guide's buffer, train, and reset the sample count.
+wl_status after_batch(wl_scene s, const wl::PtParams &P, const wl::PtBuffers &B, hipStream_t hs)
+{
+    if (!(P.gmode & WL_GUIDE_TRAIN))
+        return WL_SUCCESS;
+    wl_guide g = s->guide;
+    WL_HIP_CHECK(wl::pt_guide_collect(P, B, s->ctx->num_cus, hs));
+    wl_status st = wl::guide_train_device(g, g->dev.train, g->dev.count, g->dev.train_cap,
+                                          g->cfg.steps_per_batch, hs);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(wl::guide_roll_count(g->dev.count, g->dev.train_cap, hs));
     return WL_SUCCESS;
 }
 
@@ -110,6 +159,15 @@ wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
     P.num_lights = s->num_lights;
     std::memcpy(P.bg, s->bg, sizeof P.bg);
     P.cam = s->camera;
+    if (s->guide && s->guide_mode)
+    {
+        P.gmode = s->guide_mode;
+        P.galpha = s->guide->cfg.mix_alpha;
+        P.gfwd = s->guide->dev.fwd;
+        P.gparams = s->guide->dev.params;
+        std::memcpy(P.gbox_lo, s->box_lo, sizeof P.gbox_lo);
+        std::memcpy(P.gbox_inv, s->box_inv, sizeof P.gbox_inv);
+    }
     return P;
 }
 
@@ -175,6 +233,23 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
             }
         }
     }
+    // scene bounds (position normalisation of the guide's queries)
+    float lo[3] = { 0, 0, 0 }, hi[3] = { 1, 1, 1 };
+    if (T > 0)
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = 3.0e38f;
+            hi[k] = -3.0e38f;
+        }
+        for (size_t v = 0; v < 3 * T; ++v)
+            for (int k = 0; k < 3; ++k)
+            {
+                const float x = accel->h_tris[3 * v + k];
+                lo[k] = std::min(lo[k], x);
+                hi[k] = std::max(hi[k], x);
+            }
+    }
     std::vector<wl::PtLight> hl(std::max<uint32_t>(1, num_lights));
     for (uint32_t l = 0; l < num_lights; ++l)
     {
@@ -202,6 +277,12 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
         s->camera = *camera;
     if (bg)
         std::memcpy(s->bg, bg, sizeof s->bg);
+    for (int k = 0; k < 3; ++k)
+    {
+        s->box_lo[k] = lo[k];
+        const float ext = hi[k] - lo[k];
+        s->box_inv[k] = ext > 0.f ? 1.0f / ext : 1.0f;
+    }
     hipError_t e = hipSetDevice(ctx->device);
     auto up = [&](void **dst, const void *src, size_t bytes) {
         if (e != hipSuccess)
@@ -243,6 +324,7 @@ wl_status wl_scene_destroy(wl_scene s)
     hipFree(s->d_batch);
     hipFree(s->d_counters);
     hipFree(s->d_stats);
+    hipFree(s->d_gbatch);
     wl_context ctx = s->ctx;
     delete s;
     wl::ctx_release(ctx);
@@ -294,6 +376,9 @@ wl_status wl_pt_render(wl_scene s, const wl_pt_config *c, float *film,
             P.frame0 = c->frame_offset + s0;
             WL_HIP_CHECK(wl::pt_run_batch(P, S, B, s->ctx->num_cus, hs));
             WL_HIP_CHECK(wl::pt_accumulate(P, B, film, s->ctx->num_cus, hs));
+            st = after_batch(s, P, B, hs);
+            if (st != WL_SUCCESS)
+                return st;
         }
     }
     if (stats)
@@ -320,7 +405,7 @@ static wl_status one_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
     P.nsamples = 1;
     P.frame0 = frame;
     WL_HIP_CHECK(wl::pt_run_batch(P, scene_view(s), B, s->ctx->num_cus, hs));
-    return WL_SUCCESS;
+    return after_batch(s, P, B, hs);
 }
 
 wl_status wl_pt_render_samples(wl_scene s, const wl_pt_config *c, uint32_t frame,
@@ -384,6 +469,21 @@ wl_status wl_pt_launch_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
         P.nsamples = 1;
         WL_HIP_CHECK(wl::pt_frame_update(P, B, frame, radiance_rgba, color_rgba, hs));
     }
+    return WL_SUCCESS;
+}
+
+wl_status wl_scene_set_guide(wl_scene s, wl_guide g, uint32_t mode)
+{
+    if (!s)
+        return set_error(WL_ERROR_INVALID_VALUE, "scene is NULL");
+    if (mode & ~(uint32_t)(WL_GUIDE_SAMPLE | WL_GUIDE_TRAIN))
+        return set_error(WL_ERROR_INVALID_VALUE, "unknown guide mode bits");
+    if (g && g->ctx->device != s->ctx->device)
+        return set_error(WL_ERROR_INVALID_VALUE, "guide and scene on different devices");
+    if (!g)
+        mode = 0;
+    s->guide = mode ? g : nullptr;
+    s->guide_mode = mode;
     return WL_SUCCESS;
 }
 
