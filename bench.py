@@ -45,6 +45,10 @@ def parse():
     p.add_argument("--pt-spp", type=int, default=64,
                    help="wavefront PT 1920x1080 at this spp on the same geometry "
                         "(0 = skip)")
+    p.add_argument("--guide-spp", type=int, default=64,
+                   help="guided glossy-caustic run length in spp (0 = skip)")
+    p.add_argument("--guide-res", type=int, default=1024)
+    p.add_argument("--guide-ref-spp", type=int, default=2048)
     return p.parse_args()
 
 
@@ -80,6 +84,63 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
            "grays_per_s": paths * rays_per_path / dt / 1e9,
            "rays_per_path": rays_per_path, "seconds": dt}
     log(f"pt: {out}")
+    return out
+
+
+def relmse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def bench_guide(args, ctx, wl, torch):
+    """Learned guiding on the glossy-caustic workload (1024 x 1024, depth 16):
+    guided PT (training included in its time) against plain PT given the same
+    wall time, both scored by relMSE against a converged plain-PT reference
+    rendered on the GPU."""
+    from wayland_amd import scenes
+    res = args.guide_res
+    sc = scenes.glossy_caustic(res, res)
+    accel = wl.Accel(ctx, sc)
+    pt = wl.PathTracer(ctx, accel, sc)
+    depth = 16
+    film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
+    pt.render(wl.pt_config(res, res, spp=1, max_depth=depth), film, stats=False)  # warm
+    film.zero_()
+    t0 = time.perf_counter()
+    ref_spp = args.guide_ref_spp
+    pt.render(wl.pt_config(res, res, spp=ref_spp, max_depth=depth, frame_offset=1 << 24),
+              film, stats=False)
+    ref = pt.resolve(film, ref_spp).cpu().numpy()
+    log(f"guide: reference {ref_spp} spp in {time.perf_counter() - t0:.1f}s")
+
+    def progressive(spp, frame0):
+        cfg = wl.pt_config(res, res, spp=1, max_depth=depth, max_batch_paths=res * res)
+        f = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(spp):
+            cfg.frame_offset = frame0 + k
+            pt.render(cfg, f, stats=False)
+        torch.cuda.synchronize()
+        return pt.resolve(f, spp).cpu().numpy(), time.perf_counter() - t
+
+    guide = wl.Guide(ctx)
+    pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
+    img_g, t_g = progressive(args.guide_spp, 0)
+    ginfo = guide.info()
+    pt.set_guide(None)
+    # plain PT per-sample time, then the same wall time
+    _, t_probe = progressive(8, 1 << 20)
+    spp_p = max(1, int(round(t_g / (t_probe / 8))))
+    img_p, t_p = progressive(spp_p, 1 << 21)
+    e_g, e_p = relmse(img_g, ref), relmse(img_p, ref)
+    out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
+           "reference_spp": ref_spp,
+           "guided": {"spp": args.guide_spp, "seconds": t_g, "relmse": e_g,
+                      "adam_steps": ginfo["steps"]},
+           "plain": {"spp": spp_p, "seconds": t_p, "relmse": e_p},
+           "relmse_ratio_equal_time": e_g / e_p}
+    log(f"guide: {out}")
+    guide.close()
     return out
 
 
@@ -179,6 +240,10 @@ def main():
     if args.pt_spp > 0:
         pt_out = bench_pt(args, ctx, accel, sc, wl, torch)
 
+    guide_out = None
+    if args.guide_spp > 0:
+        guide_out = bench_guide(args, ctx, wl, torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sc.triangles(), rays_np)
@@ -212,6 +277,7 @@ def main():
                      "tris_per_ray": c[1]},
         "cpu_baseline": cpu,
         "pathtracer": pt_out,
+        "guiding": guide_out,
     }
     print(json.dumps(out), flush=True)
 

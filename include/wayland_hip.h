@@ -53,6 +53,7 @@ typedef struct wl_context_s *wl_context;
 typedef struct wl_accel_s *wl_accel;
 typedef struct wl_scene_s *wl_scene;
 typedef struct wl_guide_s *wl_guide;
+typedef struct wl_server_s *wl_server;
 
 /* ---- errors (reference: HostUtils/ErrorCheck.h:9-58, ErrorCheck.cpp:14-22) */
 const char *wl_status_string(wl_status status);
@@ -319,6 +320,59 @@ wl_status wl_guide_train(wl_guide guide, const float *d_samples, uint32_t n,
                          uint32_t steps, void *stream);
 /* Attaches (mode != 0) or detaches (guide NULL or mode 0) a guide. */
 wl_status wl_scene_set_guide(wl_scene scene, wl_guide guide, uint32_t mode);
+
+/* ---- batched serving ---------------------------------------------------------
+ * Replaces one DeviceManager::Launch per request and frame
+ * (src/Frontend/Core/DeviceManager.cpp:231-241, Launcher.cpp:12-20) for a
+ * set of independent render requests.  The server packs whole-image sample
+ * passes of the requests (in order) into megabatches of up to batch_paths
+ * paths; a megabatch's kernels are captured once into a hipGraph and the
+ * graph is replayed per megabatch, reading its work list from device memory.
+ * Each request's film equals wl_pt_render of the same request bit for bit.
+ * Requests may use different scenes of the server's device; their scenes must
+ * have no guide attached. */
+typedef struct wl_render_request
+{
+    wl_scene scene;
+    float *film;             /* device, 6 floats per pixel (as wl_pt_render) */
+    uint32_t width, height;
+    uint32_t spp, frame_offset;
+    uint32_t max_depth, rr_depth;
+    float rr_prob, epsilon;
+    uint32_t strategy, pad_;
+} wl_render_request;
+
+typedef struct wl_serve_config
+{
+    uint32_t batch_paths;          /* megabatch capacity in paths; must hold
+                                      the largest request's pixel count     */
+    uint32_t shade_blocks_per_cu;  /* shade kernel workgroups per CU        */
+    uint32_t use_graph;            /* replay a captured hipGraph            */
+    uint32_t pad_;
+} wl_serve_config;
+
+typedef struct wl_serve_stats
+{
+    double seconds;          /* GPU time from the first megabatch to the last */
+    uint64_t paths;
+    uint64_t extension_rays, shadow_rays;
+    uint32_t batches, pad_;
+} wl_serve_stats;
+
+wl_status wl_serve_default_config(wl_serve_config *cfg);
+wl_status wl_server_create(wl_context ctx, const wl_serve_config *cfg, wl_server *out);
+wl_status wl_server_destroy(wl_server server);
+wl_status wl_server_get_config(wl_server server, wl_serve_config *cfg);
+wl_status wl_server_set_config(wl_server server, const wl_serve_config *cfg);
+/* Renders n requests; waits for work already on `stream` and returns when
+ * every film is complete.  done_seconds (optional, n doubles): each request's
+ * completion time from the start of the first megabatch. */
+wl_status wl_server_render(wl_server server, const wl_render_request *reqs, uint32_t n,
+                           wl_serve_stats *stats, double *done_seconds, void *stream);
+/* Times candidate megabatch sizes and shade grids on the requests (each cut
+ * to max_spp samples, rendered into scratch films), keeps the fastest. */
+wl_status wl_server_autotune(wl_server server, const wl_render_request *reqs, uint32_t n,
+                             uint32_t max_spp, wl_serve_config *chosen);
 
 /* ---- programs, pipeline, SBT, launch: the OptiX program interface -------
  *

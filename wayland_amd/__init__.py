@@ -365,6 +365,86 @@ def mixture_pdf(ctx: Context, outs, dirs):
     return pdf
 
 
+class Server:
+    """Batched serving of many render requests (megabatches replayed from one
+    captured hipGraph).  A request is a dict: pt (PathTracer), width, height,
+    spp, and optionally frame_offset, max_depth, rr_depth, rr_prob, epsilon,
+    strategy, film (a zeroed (W*H, 6) float32 CUDA tensor)."""
+
+    def __init__(self, ctx: Context, **cfg):
+        c = _lib.ServeConfig()
+        _lib.call("wl_serve_default_config", C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        h = C.c_void_p()
+        _lib.call("wl_server_create", ctx.h, C.byref(c), C.byref(h))
+        self.ctx, self.h = ctx, h
+
+    @staticmethod
+    def _requests(reqs):
+        torch = _torch()
+        arr = (_lib.RenderRequest * len(reqs))()
+        films = []
+        for k, r in enumerate(reqs):
+            W, H = r["width"], r["height"]
+            film = r.get("film")
+            if film is None:
+                film = torch.zeros((W * H, 6), dtype=torch.float32, device="cuda")
+            films.append(film)
+            a = arr[k]
+            a.scene = r["pt"].h
+            a.film = film.data_ptr()
+            a.width, a.height, a.spp = W, H, r["spp"]
+            a.frame_offset = r.get("frame_offset", 0)
+            a.max_depth = r.get("max_depth", 25)
+            a.rr_depth = r.get("rr_depth", 5)
+            a.rr_prob = r.get("rr_prob", 0.9)
+            a.epsilon = r.get("epsilon", 1e-3)
+            a.strategy = r.get("strategy", STRATEGY_MIS)
+        return arr, films
+
+    def config(self) -> dict:
+        c = _lib.ServeConfig()
+        _lib.call("wl_server_get_config", self.h, C.byref(c))
+        return {"batch_paths": c.batch_paths, "shade_blocks_per_cu": c.shade_blocks_per_cu,
+                "use_graph": c.use_graph}
+
+    def set_config(self, **cfg):
+        c = _lib.ServeConfig()
+        _lib.call("wl_server_get_config", self.h, C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        _lib.call("wl_server_set_config", self.h, C.byref(c))
+
+    def render(self, reqs):
+        """Returns (films, stats dict, per-request completion seconds)."""
+        arr, films = self._requests(reqs)
+        st = _lib.ServeStats()
+        done = (C.c_double * len(reqs))()
+        _lib.call("wl_server_render", self.h, arr, len(reqs), C.byref(st), done,
+                  _stream_ptr())
+        stats = {k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
+        return films, stats, list(done)
+
+    def autotune(self, reqs, max_spp=4) -> dict:
+        arr, _ = self._requests([dict(r, film=None) for r in reqs])
+        c = _lib.ServeConfig()
+        _lib.call("wl_server_autotune", self.h, arr, len(reqs), int(max_spp), C.byref(c))
+        return {"batch_paths": c.batch_paths, "shade_blocks_per_cu": c.shade_blocks_per_cu,
+                "use_graph": c.use_graph}
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.load().wl_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PathTracer:
     """The PathTracing program on a scene (reference:
     src/Frontend/Programs/PathTracing/*)."""

@@ -79,6 +79,24 @@ class GuideInfo(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("last_samples", C.c_uint32), ("pad_", C.c_uint32)]
 
 
+class RenderRequest(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("film", C.c_void_p), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("spp", C.c_uint32), ("frame_offset", C.c_uint32),
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("rr_prob", C.c_float),
+                ("epsilon", C.c_float), ("strategy", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("batch_paths", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
+                ("use_graph", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("paths", C.c_uint64),
+                ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("batches", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class ProgramGroup(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("entry", (C.c_char * 64) * 3)]
 
@@ -169,6 +187,15 @@ SIGNATURES = {
     "wl_guide_gradient": (C.c_int, [_VP, _VP, _U32, _VP, _VP, _VP]),
     "wl_guide_train": (C.c_int, [_VP, _VP, _U32, _U32, _VP]),
     "wl_scene_set_guide": (C.c_int, [_VP, _VP, _U32]),
+    "wl_serve_default_config": (C.c_int, [C.POINTER(ServeConfig)]),
+    "wl_server_create": (C.c_int, [_VP, C.POINTER(ServeConfig), C.POINTER(_VP)]),
+    "wl_server_destroy": (C.c_int, [_VP]),
+    "wl_server_get_config": (C.c_int, [_VP, C.POINTER(ServeConfig)]),
+    "wl_server_set_config": (C.c_int, [_VP, C.POINTER(ServeConfig)]),
+    "wl_server_render": (C.c_int, [_VP, C.POINTER(RenderRequest), _U32, C.POINTER(ServeStats),
+                                   C.POINTER(C.c_double), _VP]),
+    "wl_server_autotune": (C.c_int, [_VP, C.POINTER(RenderRequest), _U32, _U32,
+                                     C.POINTER(ServeConfig)]),
     "wl_identify_programs": (C.c_int, [C.c_char_p, C.c_size_t, _VP, _U32, C.POINTER(_U32)]),
     "wl_pipeline_create": (C.c_int, [_VP, _VP, _U32, _U32, C.POINTER(_VP), C.POINTER(_U32)]),
     "wl_pipeline_destroy": (C.c_int, [_VP]),

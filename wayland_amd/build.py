@@ -28,6 +28,7 @@ SOURCES = [
     "wl_pt_api.cpp",
     "wl_program.cpp",
     "wl_guide_api.cpp",
+    "wl_serve.cpp",
     "traverse.hip",
     "guide.hip",
     "pathtrace.hip",

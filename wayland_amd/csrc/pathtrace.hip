@@ -82,8 +82,15 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
-template <int STACK>
-__global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int parity)
+// Request of a path in a multi-request megabatch (kept in rad.w).
+__device__ __forceinline__ uint32_t path_req(const PtBuffers &B, uint32_t path)
+{
+    return __float_as_uint(B.rad[path].w);
+}
+
+template <int STACK, bool MULTI>
+__global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int parity,
+                                                     const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[parity];
@@ -94,17 +101,29 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
         B.stats[0] += count;        // extension rays (single writer)
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
-    trav_persistent<false, kPtBlock, false>(
-        S.nodes, S.tris, count, stack + threadIdx.x,
-        [&](uint32_t i, TravRay &r) {
-            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-            const float4 a = rp[0], b = rp[1];
-            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-        },
-        [&](uint32_t i, const TravResult &h) {
-            reinterpret_cast<float4 *>(B.hits)[i] =
-                make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
-        });
+    const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
+    auto load = [&](uint32_t i, TravRay &r) {
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+        const float4 a = rp[0], b = rp[1];
+        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+    };
+    auto store = [&](uint32_t i, const TravResult &h) {
+        reinterpret_cast<float4 *>(B.hits)[i] =
+            make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+    };
+    if constexpr (MULTI)
+        trav_persistent<false, kPtBlock, false, true>(
+            nullptr, nullptr, count, stack + threadIdx.x,
+            [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
+                const PtReq &R = reqs[path_req(B, qin[i])];
+                n = R.S.nodes;
+                t = R.S.tris;
+                load(i, r);
+            },
+            store);
+    else
+        trav_persistent<false, kPtBlock, false>(S.nodes, S.tris, count, stack + threadIdx.x,
+                                                load, store);
 }
 
 // ---- guide inference for the active queue ------------------------------------
@@ -193,9 +212,11 @@ __global__ __launch_bounds__(kPtBlock) void k_guide_infer_pt(PtParams P, PtScene
 // mixing in the guide keeps the estimator's expectation unchanged.  The mixed
 // density replaces the BSDF pdf everywhere the reference uses it (the path
 // weight, lastPdf for emitter-hit MIS, the NEE MIS weight).
-template <bool GUIDED>
-__global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuffers B,
-                                                    int parity)
+// MULTI: paths of many requests; each lane shades with its request's scene
+// and parameters.
+template <bool GUIDED, bool MULTI>
+__global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P0, PtScene S0, PtBuffers B,
+                                                    int parity, const PtReq *__restrict__ reqs)
 {
     const uint32_t count = B.counters[parity];
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
@@ -211,6 +232,9 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
         if (i < count)
         {
             path = qin[i];
+            const PtReq *rq = MULTI ? reqs + path_req(B, path) : nullptr;
+            const PtParams &P = MULTI ? rq->P : P0;
+            const PtScene &S = MULTI ? rq->S : S0;
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
             const float4 s_pos = B.pos[path], s_dir = B.dir[path], s_thr = B.thr[path];
             const float4 s_ltt = B.ltt[path];
@@ -447,7 +471,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
                     }
                 }
             }
-            B.rad[path] = f4(radiance, 0.0f);
+            B.rad[path] = f4(radiance, s_rad.w); // w: request id
             if (gtrain)
             {
                 // radiance reaching the path after its first guided vertex
@@ -492,40 +516,148 @@ __global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P, PtScene S, PtBuf
 }
 
 // ---- shadow: any hit; unoccluded rays add their NEE contribution ----------
-template <int STACK>
-__global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B)
+template <int STACK, bool MULTI>
+__global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B,
+                                                     const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
-    trav_persistent<true, kPtBlock, false>(
-        S.nodes, S.tris, count, stack + threadIdx.x,
-        [&](uint32_t i, TravRay &r) {
-            const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
-            const float4 a = rp[0], b = rp[1];
-            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-        },
-        [&](uint32_t i, const TravResult &h) {
-            if (h.prim != WL_INVALID)
-                return;
-            // unoccluded: add the NEE contribution (one shadow ray per path)
-            const uint32_t path = B.spath[i];
-            const float4 c = B.scontrib[i];
-            float4 rr = B.rad[path];
-            rr.x = rr.x + c.x;
-            rr.y = rr.y + c.y;
-            rr.z = rr.z + c.z;
-            B.rad[path] = rr;
-            if (c.w != 0.0f) // the path records guide training radiance
+    auto load = [&](uint32_t i, TravRay &r) {
+        const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
+        const float4 a = rp[0], b = rp[1];
+        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+    };
+    auto add = [&](uint32_t i, const TravResult &h) {
+        if (h.prim != WL_INVALID)
+            return;
+        // unoccluded: add the NEE contribution (one shadow ray per path)
+        const uint32_t path = B.spath[i];
+        const float4 c = B.scontrib[i];
+        float4 rr = B.rad[path];
+        rr.x = rr.x + c.x;
+        rr.y = rr.y + c.y;
+        rr.z = rr.z + c.z;
+        B.rad[path] = rr;
+        if (c.w != 0.0f) // the path records guide training radiance
+        {
+            float4 ga = B.gacc[path];
+            ga.x += c.x;
+            ga.y += c.y;
+            ga.z += c.z;
+            B.gacc[path] = ga;
+        }
+    };
+    if constexpr (MULTI)
+        trav_persistent<true, kPtBlock, false, true>(
+            nullptr, nullptr, count, stack + threadIdx.x,
+            [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
+                const PtReq &R = reqs[path_req(B, B.spath[i])];
+                n = R.S.nodes;
+                t = R.S.tris;
+                load(i, r);
+            },
+            add);
+    else
+        trav_persistent<true, kPtBlock, false>(S.nodes, S.tris, count, stack + threadIdx.x,
+                                               load, add);
+}
+
+// ---- batched serving: raygen and film update of a multi-request megabatch ----
+// Item of megabatch entry x given the items' start offsets (binary search).
+__device__ __forceinline__ uint32_t find_item(const PtItem *items, uint32_t nitems, uint32_t x,
+                                              bool by_pixel)
+{
+    uint32_t lo = 0, hi = nitems; // items[lo].start <= x < items[hi].start
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint32_t st = by_pixel ? items[mid].px0 : items[mid].path0;
+        if (st <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_begin(PtMultiLaunch L)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *L.header = L.batches[*L.batch_index];
+}
+
+__global__ void k_batch_end(PtMultiLaunch L)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *L.batch_index += 1;
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_raygen_multi(PtMultiLaunch L, PtBuffers B)
+{
+    const PtBatchHeader h = *L.header;
+    const uint32_t npaths = h.paths, nitems = h.items;
+    const PtItem *items = L.items + h.item_offset;
+    for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < npaths;
+         p += gridDim.x * kPtBlock)
+    {
+        const PtItem it = items[find_item(items, nitems, p, false)];
+        const PtParams &P = L.reqs[it.req].P;
+        const uint32_t local = p - it.path0;
+        const uint32_t pixel = local % it.npix;
+        const uint32_t s = local / it.npix;
+        const uint32_t x = pixel % P.width, y = pixel / P.width;
+        uint32_t seed = tea4(pixel, it.frame0 + s);
+        // PinholeGenerateRay (Camera.h:28-35): two draws, u then v
+        const float u = 2.f * ((float)(int)x + rnd(seed)) / (float)(int)P.width - 1.f;
+        const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
+        const V3 d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
+        const V3 o = ld3(P.cam.pos);
+        B.pos[p] = f4(o, 1.0f);
+        B.dir[p] = f4(d, __uint_as_float(seed));
+        B.thr[p] = f4(mk(1, 1, 1), __uint_as_float(0u));
+        B.ltt[p] = f4(mk(1, 1, 1), 0.0f);
+        B.rad[p] = f4(mk(0, 0, 0), __uint_as_float(it.req));
+        B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
+        B.qpath0[p] = p;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        B.counters[0] = npaths;
+}
+
+// Kahan update of each item's film, samples in ascending order (the order
+// a single-request render adds them in).
+__global__ __launch_bounds__(kPtBlock) void k_accumulate_multi(PtMultiLaunch L, PtBuffers B)
+{
+    const PtBatchHeader h = *L.header;
+    const uint32_t nitems = h.items, npix = h.pixels;
+    const PtItem *items = L.items + h.item_offset;
+    for (uint32_t x = blockIdx.x * kPtBlock + threadIdx.x; x < npix; x += gridDim.x * kPtBlock)
+    {
+        const PtItem it = items[find_item(items, nitems, x, true)];
+        const uint32_t px = x - it.px0;
+        float *f = L.reqs[it.req].film + 6 * (size_t)px;
+        float sum[3] = { f[0], f[1], f[2] }, c[3] = { f[3], f[4], f[5] };
+        for (uint32_t s = 0; s < it.nsamples; ++s)
+        {
+            const float4 r = B.rad[it.path0 + (size_t)s * it.npix + px];
+            const float v[3] = { r.x, r.y, r.z };
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
             {
-                float4 ga = B.gacc[path];
-                ga.x += c.x;
-                ga.y += c.y;
-                ga.z += c.z;
-                B.gacc[path] = ga;
+                const float y = v[k] - c[k];
+                const float t = sum[k] + y;
+                c[k] = (t - sum[k]) - y;
+                sum[k] = t;
             }
-        });
+        }
+        for (int k = 0; k < 3; ++k)
+        {
+            f[k] = sum[k];
+            f[3 + k] = c[k];
+        }
+    }
 }
 
 // ---- guide training samples -----------------------------------------------------
@@ -667,11 +799,12 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     static int occ16 = 0, occ32 = 0;
     if (!occ16)
     {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ16, k_extend<16>, kPtBlock, 0) !=
-                hipSuccess || occ16 < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ16, k_extend<16, false>, kPtBlock,
+                                                         0) != hipSuccess ||
+            occ16 < 1)
             occ16 = 4;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ32, k_extend<WL_BVH8_STACK_DEPTH>,
-                                                         kPtBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ32, k_extend<WL_BVH8_STACK_DEPTH, false>, kPtBlock, 0) != hipSuccess ||
             occ32 < 1)
             occ32 = 4;
     }
@@ -686,26 +819,83 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     {
         const int parity = (int)(b & 1u);
         if (shallow)
-            hipLaunchKernelGGL(k_extend<16>, dim3(g_trav), dim3(kPtBlock), 0, st, S, B, parity);
+            hipLaunchKernelGGL((k_extend<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
+                               parity, nullptr);
         else
-            hipLaunchKernelGGL(k_extend<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
-                               S, B, parity);
+            hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, false>), dim3(g_trav),
+                               dim3(kPtBlock), 0, st, S, B, parity, nullptr);
         if (P.gmode)
         {
             hipLaunchKernelGGL(k_guide_infer_pt, dim3(g_guide), dim3(kPtBlock), 0, st, P, S, B,
                                parity);
-            hipLaunchKernelGGL(k_shade<true>, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B,
-                               parity);
+            hipLaunchKernelGGL((k_shade<true, false>), dim3(g_shade), dim3(kPtBlock), 0, st, P,
+                               S, B, parity, nullptr);
         }
         else
-            hipLaunchKernelGGL(k_shade<false>, dim3(g_shade), dim3(kPtBlock), 0, st, P, S, B,
-                               parity);
+            hipLaunchKernelGGL((k_shade<false, false>), dim3(g_shade), dim3(kPtBlock), 0, st, P,
+                               S, B, parity, nullptr);
         if (shallow)
-            hipLaunchKernelGGL(k_shadow<16>, dim3(g_trav), dim3(kPtBlock), 0, st, S, B);
+            hipLaunchKernelGGL((k_shadow<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
+                               nullptr);
         else
-            hipLaunchKernelGGL(k_shadow<WL_BVH8_STACK_DEPTH>, dim3(g_trav), dim3(kPtBlock), 0, st,
-                               S, B);
+            hipLaunchKernelGGL((k_shadow<WL_BVH8_STACK_DEPTH, false>), dim3(g_trav),
+                               dim3(kPtBlock), 0, st, S, B, nullptr);
     }
+    return hipGetLastError();
+}
+
+static int g_multi_occ16 = 0, g_multi_occ32 = 0;
+
+void pt_multi_init()
+{
+    int &occ16 = g_multi_occ16, &occ32 = g_multi_occ32;
+    if (!occ16)
+    {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ16, k_extend<16, true>, kPtBlock,
+                                                         0) != hipSuccess ||
+            occ16 < 1)
+            occ16 = 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occ32, k_extend<WL_BVH8_STACK_DEPTH, true>, kPtBlock, 0) != hipSuccess ||
+            occ32 < 1)
+            occ32 = 4;
+    }
+}
+
+hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus, hipStream_t st)
+{
+    pt_multi_init();
+    const int occ16 = g_multi_occ16, occ32 = g_multi_occ32;
+    const uint32_t blocks_all = (L.capacity + kPtBlock - 1) / kPtBlock;
+    const uint32_t cus = (uint32_t)std::max(1, num_cus);
+    const uint32_t g_gen = std::min<uint32_t>(blocks_all, cus * 8);
+    const uint32_t g_trav = std::min<uint32_t>(blocks_all, cus * (uint32_t)(L.stack16 ? occ16 : occ32));
+    const uint32_t g_shade =
+        std::min<uint32_t>(blocks_all, cus * std::max<uint32_t>(1, L.shade_blocks_per_cu));
+    const PtParams P0{};
+    const PtScene S0{};
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, L);
+    hipLaunchKernelGGL(k_raygen_multi, dim3(g_gen), dim3(kPtBlock), 0, st, L, B);
+    for (uint32_t b = 0; b < L.max_depth; ++b)
+    {
+        const int parity = (int)(b & 1u);
+        if (L.stack16)
+            hipLaunchKernelGGL((k_extend<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, S0, B,
+                               parity, L.reqs);
+        else
+            hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
+                               dim3(kPtBlock), 0, st, S0, B, parity, L.reqs);
+        hipLaunchKernelGGL((k_shade<false, true>), dim3(g_shade), dim3(kPtBlock), 0, st, P0, S0,
+                           B, parity, L.reqs);
+        if (L.stack16)
+            hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, S0, B,
+                               L.reqs);
+        else
+            hipLaunchKernelGGL((k_shadow<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
+                               dim3(kPtBlock), 0, st, S0, B, L.reqs);
+    }
+    hipLaunchKernelGGL(k_accumulate_multi, dim3(g_gen), dim3(kPtBlock), 0, st, L, B);
+    hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, L);
     return hipGetLastError();
 }
 

@@ -28,13 +28,17 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 }
 
 // fetch(i, TravRay&) sets up ray i (i < n); emit(i, const TravResult&)
-// consumes its result.  BLOCK must be a multiple of 64.
-template <bool ANY, int BLOCK, bool AUX, class Fetch, class Emit>
+// consumes its result.  BLOCK must be a multiple of 64.  With PERLANE the
+// rays of one launch may belong to different BVHs: fetch(i, TravRay&,
+// nodes&, tris&) also sets the lane's node and triangle arrays.
+template <bool ANY, int BLOCK, bool AUX, bool PERLANE = false, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
                                                 const WlTriRecord *__restrict__ tris,
                                                 uint32_t n, uint32_t *stack_lane,
                                                 Fetch fetch, Emit emit)
 {
+    const WlBvh8Node *ln = nodes;
+    const WlTriRecord *lt = tris;
     static_assert(BLOCK % 64 == 0, "whole waves only");
     const uint32_t waves = gridDim.x * (BLOCK / 64);
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64;
@@ -61,7 +65,10 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                 if (k < end)
                 {
                     rid = k;
-                    fetch(rid, r);
+                    if constexpr (PERLANE)
+                        fetch(rid, r, ln, lt);
+                    else
+                        fetch(rid, r);
                     trav_init(st, r, true);
                     alive = true;
                 }
@@ -74,7 +81,7 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                 break;
             continue;
         }
-        if (alive && trav_step<ANY, false, BLOCK, AUX>(nodes, tris, r, st, stack_lane, nn, nt))
+        if (alive && trav_step<ANY, false, BLOCK, AUX>(ln, lt, r, st, stack_lane, nn, nt))
         {
             trav_finish(st.res);
             emit(rid, st.res);

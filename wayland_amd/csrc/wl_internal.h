@@ -99,6 +99,15 @@ hipError_t launch_trace_closest(const wl_accel_s *a, const wl_ray *rays,
                                 hipStream_t s);
 hipError_t launch_trace_any(const wl_accel_s *a, const wl_ray *rays,
                             uint32_t n, uint32_t *bits, hipStream_t s);
+// path-tracer helpers shared by wl_pt_api.cpp and wl_serve.cpp
+struct PtBuffers;
+struct PtScene;
+struct PtParams;
+PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long long *stats);
+size_t pt_batch_bytes(size_t cap);
+PtScene pt_scene_view(wl_scene s);
+wl_status pt_check_config(wl_scene s, const wl_pt_config *c);
+PtParams pt_params_for(wl_scene s, const wl_pt_config *c);
 // Adam steps of a guide over samples[0, min(*n_dev, n_cap)) (wl_guide_api.cpp)
 wl_status guide_train_device(wl_guide g, const float *samples, const uint32_t *n_dev,
                              uint32_t n_cap, uint32_t steps, hipStream_t s);

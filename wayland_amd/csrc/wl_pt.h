@@ -93,6 +93,56 @@ struct PtBuffers
 
 hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                         int num_cus, hipStream_t st);
+
+// ---- batched serving: one megabatch mixes paths of many requests ----------
+// A request's scene, parameters and film; a path finds its request through
+// the id kept in rad.w.
+struct PtReq
+{
+    PtParams P; // pix0 / npix / nsamples / frame0 unused; gmode 0
+    PtScene S;
+    float *film; // 6 floats per pixel (Kahan sum + compensation)
+    uint32_t pad_[2];
+};
+
+// Work item: all pixels of request `req` for samples [frame0, frame0 +
+// nsamples); its paths are [path0, path0 + npix * nsamples) of the megabatch
+// (sample-major like a single-request batch), its pixels [px0, px0 + npix)
+// of the megabatch's pixel list.  A megabatch holds each request at most
+// once, so the film update of a pixel never races.
+struct PtItem
+{
+    uint32_t req, npix, frame0, nsamples;
+    uint32_t path0, px0, pad_[2];
+};
+
+// Megabatch headers: PtBatchHeader per batch; the graph's first kernel
+// copies the header of batch *batch_index into `header`, its last kernel
+// advances *batch_index, so consecutive replays walk the batch list.
+struct PtBatchHeader
+{
+    uint32_t paths, items, pixels, item_offset;
+};
+
+struct PtMultiLaunch
+{
+    const PtReq *reqs;
+    const PtItem *items;          // all batches' items
+    const PtBatchHeader *batches; // all batches' headers
+    uint32_t *batch_index;        // device counter of the batch being run
+    PtBatchHeader *header;        // the current batch's header
+    uint32_t max_depth;  // bounces enqueued (maximum over the requests)
+    uint32_t stack16;    // every BVH at most 16 levels deep
+    uint32_t capacity;   // paths the buffers hold
+    uint32_t shade_blocks_per_cu;
+};
+// Enqueues raygen, max_depth x (extend, shade, shadow) and the film update
+// of one megabatch; every count comes from device memory, so the sequence
+// can be captured once into a hipGraph and replayed.
+hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
+                        hipStream_t st);
+// One-time occupancy queries of the multi-request kernels (outside capture).
+void pt_multi_init();
 // Moves the finished batch's training records with a positive weight into
 // the training buffer.
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus,

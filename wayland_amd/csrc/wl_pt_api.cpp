@@ -15,12 +15,15 @@ using wl::set_error;
 namespace
 {
 
-// Carves the batch buffers for `cap` paths out of one allocation.
-wl::PtBuffers carve(wl_scene s)
+} // namespace
+
+namespace wl
 {
-    wl::PtBuffers B{};
-    const size_t cap = s->capacity;
-    char *p = (char *)s->d_batch;
+// Carves the batch buffers for `cap` paths out of one allocation.
+PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long long *stats)
+{
+    PtBuffers B{};
+    char *p = (char *)base;
     auto take = [&](size_t bytes) {
         char *q = p;
         p += (bytes + 255) & ~(size_t)255;
@@ -39,8 +42,73 @@ wl::PtBuffers carve(wl_scene s)
     B.srays = (wl_ray *)take(32 * cap);
     B.spath = (uint32_t *)take(4 * cap);
     B.scontrib = (float4 *)take(16 * cap);
-    B.counters = s->d_counters;
-    B.stats = s->d_stats;
+    B.counters = counters;
+    B.stats = stats;
+    return B;
+}
+
+size_t pt_batch_bytes(size_t cap)
+{
+    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 3 + r(16 * cap) * 2;
+}
+
+PtScene pt_scene_view(wl_scene s)
+{
+    PtScene S;
+    S.nodes = s->accel->d_nodes;
+    S.tris = s->accel->d_tris;
+    S.prims = s->d_prims;
+    S.normals = s->d_normals;
+    S.meshes = (const PtMesh *)s->d_meshes;
+    S.materials = s->d_materials;
+    S.lights = (const PtLight *)s->d_lights;
+    S.bvh_depth = s->accel->info.depth;
+    return S;
+}
+
+wl_status pt_check_config(wl_scene s, const wl_pt_config *c)
+{
+    if (!s || !c)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (c->width == 0 || c->height == 0)
+        return set_error(WL_ERROR_INVALID_VALUE, "empty image");
+    if ((uint64_t)c->width * c->height > 0x7FFFFFFFull)
+        return set_error(WL_ERROR_LIMIT_EXCEEDED, "image too large");
+    if (c->max_depth == 0 || c->max_depth > 1024)
+        return set_error(WL_ERROR_INVALID_VALUE, "max_depth must be in [1, 1024]");
+    if (!(c->rr_prob > 0.0f && c->rr_prob <= 1.0f))
+        return set_error(WL_ERROR_INVALID_VALUE, "rr_prob must be in (0, 1]");
+    if (c->strategy > WL_STRATEGY_MIS)
+        return set_error(WL_ERROR_INVALID_VALUE, "unknown strategy");
+    return WL_SUCCESS;
+}
+
+// Kernel parameters of a render of `s` with `c` (no guide).
+PtParams pt_params_for(wl_scene s, const wl_pt_config *c)
+{
+    PtParams P{};
+    P.width = c->width;
+    P.height = c->height;
+    P.max_depth = c->max_depth;
+    P.rr_depth = c->rr_depth;
+    P.rr_prob = c->rr_prob;
+    P.epsilon = c->epsilon;
+    P.strategy = c->strategy;
+    P.num_lights = s->num_lights;
+    std::memcpy(P.bg, s->bg, sizeof P.bg);
+    P.cam = s->camera;
+    return P;
+}
+} // namespace wl
+
+namespace
+{
+
+wl::PtBuffers carve(wl_scene s)
+{
+    wl::PtBuffers B = wl::pt_carve(s->d_batch, s->capacity, s->d_counters, s->d_stats);
+    const size_t cap = s->capacity;
     if (s->guide && s->guide_mode && s->d_gbatch)
     {
         char *g = (char *)s->d_gbatch;
@@ -58,11 +126,7 @@ wl::PtBuffers carve(wl_scene s)
     return B;
 }
 
-size_t batch_bytes(size_t cap)
-{
-    auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 3 + r(16 * cap) * 2;
-}
+size_t batch_bytes(size_t cap) { return wl::pt_batch_bytes(cap); }
 
 size_t guide_batch_bytes(size_t cap)
 {
@@ -115,50 +179,13 @@ wl_status after_batch(wl_scene s, const wl::PtParams &P, const wl::PtBuffers &B,
     return WL_SUCCESS;
 }
 
-wl::PtScene scene_view(wl_scene s)
-{
-    wl::PtScene S;
-    S.nodes = s->accel->d_nodes;
-    S.tris = s->accel->d_tris;
-    S.prims = s->d_prims;
-    S.normals = s->d_normals;
-    S.meshes = (const wl::PtMesh *)s->d_meshes;
-    S.materials = s->d_materials;
-    S.lights = (const wl::PtLight *)s->d_lights;
-    S.bvh_depth = s->accel->info.depth;
-    return S;
-}
+wl::PtScene scene_view(wl_scene s) { return wl::pt_scene_view(s); }
 
-wl_status check_config(wl_scene s, const wl_pt_config *c)
-{
-    if (!s || !c)
-        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
-    if (c->width == 0 || c->height == 0)
-        return set_error(WL_ERROR_INVALID_VALUE, "empty image");
-    if ((uint64_t)c->width * c->height > 0x7FFFFFFFull)
-        return set_error(WL_ERROR_LIMIT_EXCEEDED, "image too large");
-    if (c->max_depth == 0 || c->max_depth > 1024)
-        return set_error(WL_ERROR_INVALID_VALUE, "max_depth must be in [1, 1024]");
-    if (!(c->rr_prob > 0.0f && c->rr_prob <= 1.0f))
-        return set_error(WL_ERROR_INVALID_VALUE, "rr_prob must be in (0, 1]");
-    if (c->strategy > WL_STRATEGY_MIS)
-        return set_error(WL_ERROR_INVALID_VALUE, "unknown strategy");
-    return WL_SUCCESS;
-}
+wl_status check_config(wl_scene s, const wl_pt_config *c) { return wl::pt_check_config(s, c); }
 
 wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
 {
-    wl::PtParams P{};
-    P.width = c->width;
-    P.height = c->height;
-    P.max_depth = c->max_depth;
-    P.rr_depth = c->rr_depth;
-    P.rr_prob = c->rr_prob;
-    P.epsilon = c->epsilon;
-    P.strategy = c->strategy;
-    P.num_lights = s->num_lights;
-    std::memcpy(P.bg, s->bg, sizeof P.bg);
-    P.cam = s->camera;
+    wl::PtParams P = wl::pt_params_for(s, c);
     if (s->guide && s->guide_mode)
     {
         P.gmode = s->guide_mode;

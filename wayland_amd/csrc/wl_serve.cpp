@@ -1,0 +1,503 @@
+// wl_serve.cpp — batched serving: many render requests (each a scene, an
+// image size, a sample count and path-tracer parameters) in shared
+// megabatches.  The reference renders one launch per request and frame
+// (DeviceManager::Launch, src/Frontend/Core/DeviceManager.cpp:231-241);
+// here the scheduler packs whole requests x sample ranges into megabatches
+// of up to batch_paths paths, each megabatch's kernels (raygen, max_depth x
+// extend / shade / shadow, film update) are captured once into a hipGraph,
+// and the graph is replayed once per megabatch: the batch descriptors live in
+// device memory and the graph's first kernel picks the next one, so the host
+// enqueues nothing but graph launches.  An autotuner times candidate
+// megabatch sizes and shade grids on the workload itself.
+//
+// Each request's film equals its standalone wl_pt_render result bit for bit:
+// a path's arithmetic depends only on its own state and its request's scene,
+// and every pixel receives its samples in ascending order.
+#include "wl_internal.h"
+#include "wl_pt.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using wl::set_error;
+
+struct wl_server_s
+{
+    wl_context ctx = nullptr;
+    wl_serve_config cfg{};
+    hipStream_t stream = nullptr; // capture and replay stream
+    // batch buffers
+    void *d_buf = nullptr;
+    uint32_t capacity = 0;
+    uint32_t *d_counters = nullptr;
+    unsigned long long *d_stats = nullptr;
+    // request / batch descriptors
+    wl::PtReq *d_reqs = nullptr;
+    uint32_t reqs_cap = 0;
+    wl::PtItem *d_items = nullptr;
+    uint32_t items_cap = 0;
+    wl::PtBatchHeader *d_batches = nullptr;
+    uint32_t batches_cap = 0;
+    uint32_t *d_batch_index = nullptr;
+    wl::PtBatchHeader *d_header = nullptr;
+    // captured megabatch
+    hipGraphExec_t exec = nullptr;
+    uint64_t graph_key = 0;
+    std::vector<hipEvent_t> events;
+};
+
+namespace
+{
+
+constexpr uint32_t kDefaultServeBatch = 1u << 22;
+
+template <class T>
+wl_status grow(T **p, uint32_t &cap, uint32_t need)
+{
+    if (need <= cap)
+        return WL_SUCCESS;
+    if (*p)
+    {
+        WL_HIP_CHECK(hipDeviceSynchronize());
+        WL_HIP_CHECK(hipFree(*p));
+        *p = nullptr;
+        cap = 0;
+    }
+    const uint32_t n = std::max<uint32_t>(need, 64);
+    WL_HIP_CHECK(hipMalloc((void **)p, sizeof(T) * (size_t)n));
+    cap = n;
+    return WL_SUCCESS;
+}
+
+void drop_graph(wl_server s)
+{
+    if (s->exec)
+    {
+        hipGraphExecDestroy(s->exec);
+        s->exec = nullptr;
+    }
+    s->graph_key = 0;
+}
+
+wl_status ensure_buffers(wl_server s, uint32_t capacity)
+{
+    if (capacity == s->capacity && s->d_buf)
+        return WL_SUCCESS;
+    drop_graph(s);
+    if (s->d_buf)
+    {
+        WL_HIP_CHECK(hipDeviceSynchronize());
+        WL_HIP_CHECK(hipFree(s->d_buf));
+        s->d_buf = nullptr;
+        s->capacity = 0;
+    }
+    WL_HIP_CHECK(hipMalloc(&s->d_buf, wl::pt_batch_bytes(capacity)));
+    s->capacity = capacity;
+    return WL_SUCCESS;
+}
+
+wl_pt_config request_config(const wl_render_request &r)
+{
+    wl_pt_config c{};
+    c.width = r.width;
+    c.height = r.height;
+    c.spp = r.spp;
+    c.frame_offset = r.frame_offset;
+    c.max_depth = r.max_depth;
+    c.rr_depth = r.rr_depth;
+    c.rr_prob = r.rr_prob;
+    c.epsilon = r.epsilon;
+    c.strategy = r.strategy;
+    return c;
+}
+
+wl_status normalise_config(wl_serve_config &c)
+{
+    if (c.batch_paths == 0)
+        c.batch_paths = kDefaultServeBatch;
+    if (c.shade_blocks_per_cu == 0)
+        c.shade_blocks_per_cu = 8;
+    if (c.batch_paths > (1u << 26))
+        return set_error(WL_ERROR_LIMIT_EXCEEDED, "batch_paths above 2^26");
+    if (c.shade_blocks_per_cu > 64)
+        return set_error(WL_ERROR_INVALID_VALUE, "shade_blocks_per_cu above 64");
+    return WL_SUCCESS;
+}
+
+struct Plan
+{
+    std::vector<wl::PtItem> items;
+    std::vector<wl::PtBatchHeader> batches;
+    std::vector<uint32_t> last_batch; // per request
+    uint64_t paths = 0;
+};
+
+// FIFO packing: each megabatch takes, in request order, as many whole-image
+// sample passes of every unfinished request as still fit.
+void plan(const wl_render_request *r, uint32_t n, uint32_t capacity, Plan &P)
+{
+    std::vector<uint32_t> rem(n), next(n, 0);
+    P.last_batch.assign(n, 0);
+    uint32_t pending = 0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        rem[k] = r[k].spp;
+        pending += rem[k] > 0;
+    }
+    while (pending)
+    {
+        wl::PtBatchHeader h{ 0, 0, 0, (uint32_t)P.items.size() };
+        uint32_t free_paths = capacity;
+        for (uint32_t k = 0; k < n && free_paths; ++k)
+        {
+            if (!rem[k])
+                continue;
+            const uint32_t npix = r[k].width * r[k].height;
+            if (npix > free_paths)
+                continue;
+            const uint32_t ns = std::min(rem[k], free_paths / npix);
+            wl::PtItem it{};
+            it.req = k;
+            it.npix = npix;
+            it.frame0 = r[k].frame_offset + next[k];
+            it.nsamples = ns;
+            it.path0 = h.paths;
+            it.px0 = h.pixels;
+            P.items.push_back(it);
+            h.paths += ns * npix;
+            h.pixels += npix;
+            h.items += 1;
+            free_paths -= ns * npix;
+            next[k] += ns;
+            rem[k] -= ns;
+            P.last_batch[k] = (uint32_t)P.batches.size();
+            if (!rem[k])
+                --pending;
+        }
+        P.paths += h.paths;
+        P.batches.push_back(h);
+    }
+}
+
+wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_stats *stats,
+              double *done_seconds, hipStream_t user)
+{
+    if (!s || (n && !reqs))
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (n == 0)
+        return WL_SUCCESS;
+    wl_serve_config cfg = s->cfg;
+    wl_status st = normalise_config(cfg);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    std::vector<wl::PtReq> hreq(n);
+    uint32_t max_npix = 0, max_depth = 0, max_bvh = 0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const wl_render_request &r = reqs[k];
+        if (!r.scene || !r.film)
+            return set_error(WL_ERROR_INVALID_VALUE, "request without scene or film");
+        if (r.scene->ctx->device != s->ctx->device)
+            return set_error(WL_ERROR_INVALID_VALUE, "request scene on another device");
+        if (r.scene->guide && r.scene->guide_mode)
+            return set_error(WL_ERROR_NOT_SUPPORTED, "batched serving of a guided scene");
+        const wl_pt_config c = request_config(r);
+        st = wl::pt_check_config(r.scene, &c);
+        if (st != WL_SUCCESS)
+            return st;
+        hreq[k].P = wl::pt_params_for(r.scene, &c);
+        hreq[k].S = wl::pt_scene_view(r.scene);
+        hreq[k].film = r.film;
+        max_npix = std::max(max_npix, r.width * r.height);
+        max_depth = std::max(max_depth, r.max_depth);
+        max_bvh = std::max(max_bvh, r.scene->accel->info.depth);
+    }
+    if (max_npix > cfg.batch_paths)
+        return set_error(WL_ERROR_LIMIT_EXCEEDED,
+                         "batch_paths must hold the largest request's pixel count");
+    Plan pl;
+    plan(reqs, n, cfg.batch_paths, pl);
+    const uint32_t nb = (uint32_t)pl.batches.size();
+    st = ensure_buffers(s, cfg.batch_paths);
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_reqs, s->reqs_cap, n);
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_items, s->items_cap, (uint32_t)pl.items.size());
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_batches, s->batches_cap, nb);
+    if (st != WL_SUCCESS)
+        return st;
+    // the descriptors of this call; the graph reads them through fixed pointers
+    WL_HIP_CHECK(hipMemcpy(s->d_reqs, hreq.data(), sizeof(wl::PtReq) * n, hipMemcpyHostToDevice));
+    WL_HIP_CHECK(hipMemcpy(s->d_items, pl.items.data(), sizeof(wl::PtItem) * pl.items.size(),
+                           hipMemcpyHostToDevice));
+    WL_HIP_CHECK(hipMemcpy(s->d_batches, pl.batches.data(), sizeof(wl::PtBatchHeader) * nb,
+                           hipMemcpyHostToDevice));
+
+    wl::PtBuffers B = wl::pt_carve(s->d_buf, s->capacity, s->d_counters, s->d_stats);
+    wl::PtMultiLaunch L{};
+    L.reqs = s->d_reqs;
+    L.items = s->d_items;
+    L.batches = s->d_batches;
+    L.batch_index = s->d_batch_index;
+    L.header = s->d_header;
+    L.max_depth = max_depth;
+    L.stack16 = max_bvh <= 16 ? 1u : 0u;
+    L.capacity = s->capacity;
+    L.shade_blocks_per_cu = cfg.shade_blocks_per_cu;
+
+    hipEvent_t ready;
+    WL_HIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    WL_HIP_CHECK(hipEventRecord(ready, user));
+    WL_HIP_CHECK(hipStreamWaitEvent(s->stream, ready, 0));
+    WL_HIP_CHECK(hipEventDestroy(ready));
+
+    if (cfg.use_graph)
+    {
+        // the graph bakes in buffer pointers, grids and the bounce count
+        const uint64_t key = ((uint64_t)max_depth << 48) ^ ((uint64_t)L.stack16 << 47) ^
+                             ((uint64_t)cfg.shade_blocks_per_cu << 40) ^ (uint64_t)s->capacity ^
+                             ((uint64_t)(uintptr_t)s->d_reqs << 1) ^
+                             ((uint64_t)(uintptr_t)s->d_items << 3) ^
+                             ((uint64_t)(uintptr_t)s->d_batches << 5) | 1ull;
+        if (!s->exec || s->graph_key != key)
+        {
+            drop_graph(s);
+            wl::pt_multi_init();
+            hipGraph_t g = nullptr;
+            WL_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e1 = wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream);
+            const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess)
+            {
+                if (g)
+                    hipGraphDestroy(g);
+                return wl::hip_error(e1 != hipSuccess ? e1 : e2, "megabatch capture");
+            }
+            const hipError_t e3 = hipGraphInstantiateWithFlags(&s->exec, g, 0);
+            hipGraphDestroy(g);
+            if (e3 != hipSuccess)
+            {
+                s->exec = nullptr;
+                return wl::hip_error(e3, "megabatch graph instantiate");
+            }
+            s->graph_key = key;
+        }
+    }
+    while (s->events.size() < (size_t)nb + 1)
+    {
+        hipEvent_t e;
+        WL_HIP_CHECK(hipEventCreate(&e));
+        s->events.push_back(e);
+    }
+    // counters: zeroed by a plain copy before the replays (never inside the graph)
+    const uint32_t zeros[16] = {};
+    WL_HIP_CHECK(hipMemcpyAsync(s->d_batch_index, zeros, 4, hipMemcpyHostToDevice, s->stream));
+    WL_HIP_CHECK(hipMemcpyAsync(s->d_stats, zeros, 16, hipMemcpyHostToDevice, s->stream));
+    WL_HIP_CHECK(hipEventRecord(s->events[0], s->stream));
+    for (uint32_t b = 0; b < nb; ++b)
+    {
+        if (cfg.use_graph)
+            WL_HIP_CHECK(hipGraphLaunch(s->exec, s->stream));
+        else
+            WL_HIP_CHECK(wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream));
+        WL_HIP_CHECK(hipEventRecord(s->events[b + 1], s->stream));
+    }
+    WL_HIP_CHECK(hipStreamSynchronize(s->stream));
+    std::vector<double> tb(nb + 1, 0.0);
+    for (uint32_t b = 1; b <= nb; ++b)
+    {
+        float ms = 0.f;
+        WL_HIP_CHECK(hipEventElapsedTime(&ms, s->events[0], s->events[b]));
+        tb[b] = ms * 1e-3;
+    }
+    if (done_seconds)
+        for (uint32_t k = 0; k < n; ++k)
+            done_seconds[k] = reqs[k].spp ? tb[pl.last_batch[k] + 1] : 0.0;
+    if (stats)
+    {
+        unsigned long long h[2] = { 0, 0 };
+        WL_HIP_CHECK(hipMemcpy(h, s->d_stats, 16, hipMemcpyDeviceToHost));
+        std::memset(stats, 0, sizeof *stats);
+        stats->seconds = tb[nb];
+        stats->paths = pl.paths;
+        stats->extension_rays = h[0];
+        stats->shadow_rays = h[1];
+        stats->batches = nb;
+    }
+    return WL_SUCCESS;
+}
+
+} // namespace
+
+extern "C" {
+
+wl_status wl_serve_default_config(wl_serve_config *cfg)
+{
+    if (!cfg)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->batch_paths = kDefaultServeBatch;
+    cfg->shade_blocks_per_cu = 8;
+    cfg->use_graph = 1;
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_create(wl_context ctx, const wl_serve_config *cfg, wl_server *out)
+{
+    if (!ctx || !out)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    *out = nullptr;
+    wl_serve_config c;
+    if (cfg)
+        c = *cfg;
+    else
+        wl_serve_default_config(&c);
+    wl_status st = normalise_config(c);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(hipSetDevice(ctx->device));
+    wl_server s = new (std::nothrow) wl_server_s;
+    if (!s)
+        return set_error(WL_ERROR_OUT_OF_MEMORY, "server allocation");
+    s->ctx = ctx;
+    s->cfg = c;
+    wl::ctx_retain(ctx);
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_counters, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_stats, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_batch_index, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_header, 64);
+    if (e != hipSuccess)
+    {
+        wl_server_destroy(s);
+        return wl::hip_error(e, "server set-up");
+    }
+    *out = s;
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_destroy(wl_server s)
+{
+    if (!s)
+        return set_error(WL_ERROR_INVALID_VALUE, "server is NULL");
+    hipSetDevice(s->ctx->device);
+    hipDeviceSynchronize();
+    drop_graph(s);
+    for (hipEvent_t e : s->events)
+        hipEventDestroy(e);
+    hipFree(s->d_buf);
+    hipFree(s->d_counters);
+    hipFree(s->d_stats);
+    hipFree(s->d_reqs);
+    hipFree(s->d_items);
+    hipFree(s->d_batches);
+    hipFree(s->d_batch_index);
+    hipFree(s->d_header);
+    if (s->stream)
+        hipStreamDestroy(s->stream);
+    wl_context ctx = s->ctx;
+    delete s;
+    wl::ctx_release(ctx);
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_get_config(wl_server s, wl_serve_config *cfg)
+{
+    if (!s || !cfg)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    *cfg = s->cfg;
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_set_config(wl_server s, const wl_serve_config *cfg)
+{
+    if (!s || !cfg)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    wl_serve_config c = *cfg;
+    wl_status st = normalise_config(c);
+    if (st != WL_SUCCESS)
+        return st;
+    s->cfg = c;
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_render(wl_server s, const wl_render_request *reqs, uint32_t n,
+                           wl_serve_stats *stats, double *done_seconds, void *stream)
+{
+    return run(s, reqs, n, stats, done_seconds, (hipStream_t)stream);
+}
+
+wl_status wl_server_autotune(wl_server s, const wl_render_request *reqs, uint32_t n,
+                             uint32_t max_spp, wl_serve_config *chosen)
+{
+    if (!s || !reqs || n == 0)
+        return set_error(WL_ERROR_INVALID_VALUE, "bad argument");
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    // the workload with every request cut to max_spp samples, into scratch films
+    std::vector<wl_render_request> probe(reqs, reqs + n);
+    size_t film_floats = 0;
+    uint32_t max_npix = 0;
+    for (auto &r : probe)
+    {
+        r.spp = std::min(r.spp, std::max<uint32_t>(1, max_spp));
+        film_floats += 6 * (size_t)r.width * r.height;
+        max_npix = std::max(max_npix, r.width * r.height);
+    }
+    float *scratch = nullptr;
+    WL_HIP_CHECK(hipMalloc((void **)&scratch, 4 * film_floats));
+    size_t off = 0;
+    for (auto &r : probe)
+    {
+        r.film = scratch + off;
+        off += 6 * (size_t)r.width * r.height;
+    }
+    const wl_serve_config saved = s->cfg;
+    wl_serve_config best = saved;
+    double best_rate = -1.0;
+    const uint32_t batches[] = { 1u << 20, 1u << 21, 1u << 22, 1u << 23 };
+    const uint32_t shades[] = { 4, 8, 16 };
+    wl_status st = WL_SUCCESS;
+    for (uint32_t bp : batches)
+    {
+        if (bp < max_npix)
+            continue;
+        for (uint32_t sh : shades)
+        {
+            wl_serve_config c = saved;
+            c.batch_paths = bp;
+            c.shade_blocks_per_cu = sh;
+            c.use_graph = 1;
+            s->cfg = c;
+            wl_serve_stats a{}, b{};
+            st = run(s, probe.data(), n, &a, nullptr, nullptr); // warm (capture)
+            if (st == WL_SUCCESS)
+                st = run(s, probe.data(), n, &b, nullptr, nullptr);
+            if (st != WL_SUCCESS)
+                break;
+            const double rate = (double)b.paths / std::max(1e-9, b.seconds);
+            if (rate > best_rate)
+            {
+                best_rate = rate;
+                best = c;
+            }
+        }
+        if (st != WL_SUCCESS)
+            break;
+    }
+    hipFree(scratch);
+    s->cfg = st == WL_SUCCESS && best_rate > 0 ? best : saved;
+    if (chosen)
+        *chosen = s->cfg;
+    return st;
+}
+
+} // extern "C"
