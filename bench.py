@@ -49,6 +49,9 @@ def parse():
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
     p.add_argument("--guide-ref-spp", type=int, default=2048)
+    p.add_argument("--cornell-ref-spp", type=int, default=256,
+                   help="FP64 oracle samples per pixel for the Cornell relMSE "
+                        "(0 = skip the Cornell leg)")
     return p.parse_args()
 
 
@@ -87,8 +90,13 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
     return out
 
 
-def relmse(img, ref):
-    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+def relmse(img, ref, trim=0.0):
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01); trim > 0
+    drops that fraction of the largest per-pixel errors (fireflies)."""
+    e = ((img - ref) ** 2 / (ref ** 2 + 0.01)).reshape(-1, 3).mean(1)
+    if trim > 0:
+        e = np.sort(e)[: e.size - int(e.size * trim)]
+    return float(e.mean())
 
 
 def bench_guide(args, ctx, wl, torch):
@@ -133,14 +141,70 @@ def bench_guide(args, ctx, wl, torch):
     spp_p = max(1, int(round(t_g / (t_probe / 8))))
     img_p, t_p = progressive(spp_p, 1 << 21)
     e_g, e_p = relmse(img_g, ref), relmse(img_p, ref)
-    out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
-           "reference_spp": ref_spp,
+    t_g1, t_p1 = relmse(img_g, ref, 1e-3), relmse(img_p, ref, 1e-3)
+    out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}, "
+                       "overhead view",
+           "reference": f"plain PT {ref_spp} spp on the GPU",
            "guided": {"spp": args.guide_spp, "seconds": t_g, "relmse": e_g,
-                      "adam_steps": ginfo["steps"]},
-           "plain": {"spp": spp_p, "seconds": t_p, "relmse": e_p},
-           "relmse_ratio_equal_time": e_g / e_p}
+                      "relmse_trim_0.1pct": t_g1, "adam_steps": ginfo["steps"]},
+           "plain": {"spp": spp_p, "seconds": t_p, "relmse": e_p,
+                     "relmse_trim_0.1pct": t_p1},
+           "relmse_ratio_equal_time": e_g / e_p,
+           "relmse_trim_ratio_equal_time": t_g1 / t_p1}
     log(f"guide: {out}")
     guide.close()
+    return out
+
+
+def bench_cornell(args, ctx, wl, torch):
+    """Plain-PT CPU workload on both sides: Cornell box 256 x 256, 16 spp,
+    depth 8, NEE + MIS.  GPU render vs the oracle's host restatement of
+    PathTracing.cu on the host cores (same TEA seeds), both scored by relMSE
+    against the oracle's FP64 estimator at args.cornell_ref_spp samples."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+    from wayland_amd import scenes
+    res, spp, depth = 256, 16, 8
+    sc = scenes.cornell_box(res, res)
+    accel = wl.Accel(ctx, sc)
+    pt = wl.PathTracer(ctx, accel, sc)
+    cfg = wl.pt_config(res, res, spp=spp, max_depth=depth)
+    pt.render(cfg, stats=False)  # warm
+    times = []
+    for _ in range(5):
+        film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pt.render(cfg, film, stats=False)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    t_gpu = float(np.median(times))
+    img_gpu = pt.resolve(film, spp).cpu().numpy().reshape(res, res, 3)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    opt = ob.OraclePT(sc)
+    ocfg = opt.config(res, res, max_depth=depth)
+    opt.render_frame(ocfg, 0, nthreads)  # warm
+    t = time.perf_counter()
+    acc = np.zeros((res, res, 3), np.float64)
+    for f in range(spp):
+        img, _ = opt.render_frame(ocfg, f, nthreads)
+        acc += img
+    t_cpu = time.perf_counter() - t
+    img_cpu = acc / spp
+    t = time.perf_counter()
+    ref = opt.render_f64(ocfg, 1 << 20, args.cornell_ref_spp, nthreads)
+    t_ref = time.perf_counter() - t
+    paths = res * res * spp
+    out = {"workload": f"cornell box {res}x{res}, {spp} spp, depth {depth}, NEE+MIS, FP32, "
+                       "TEA/LCG seeds (pixel, frame) as PathTracing.cu",
+           "gpu_mpaths_per_s": paths / t_gpu / 1e6, "gpu_seconds": t_gpu,
+           "cpu_mpaths_per_s": paths / t_cpu / 1e6, "cpu_seconds": t_cpu,
+           "cpu_threads": nthreads, "cpu_kind": "port",
+           "relmse_gpu_vs_fp64_oracle": relmse(img_gpu, ref),
+           "relmse_cpu_vs_fp64_oracle": relmse(img_cpu, ref),
+           "max_abs_gpu_minus_cpu": float(np.abs(img_gpu - img_cpu).max()),
+           "fp64_oracle_spp": args.cornell_ref_spp, "fp64_oracle_seconds": t_ref}
+    log(f"cornell: {out}")
     return out
 
 
@@ -240,6 +304,10 @@ def main():
     if args.pt_spp > 0:
         pt_out = bench_pt(args, ctx, accel, sc, wl, torch)
 
+    cornell_out = None
+    if args.cornell_ref_spp > 0:
+        cornell_out = bench_cornell(args, ctx, wl, torch)
+
     guide_out = None
     if args.guide_spp > 0:
         guide_out = bench_guide(args, ctx, wl, torch)
@@ -278,6 +346,7 @@ def main():
         "cpu_baseline": cpu,
         "pathtracer": pt_out,
         "guiding": guide_out,
+        "cornell_cpu_workload": cornell_out,
     }
     print(json.dumps(out), flush=True)
 

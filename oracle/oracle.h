@@ -162,6 +162,14 @@ void oracle_pt_free(oracle_pt *pt);
 uint32_t oracle_pt_sample_f32(const oracle_pt *pt, const oracle_pt_config *c,
                               uint32_t px, uint32_t py, uint32_t frame,
                               float rgb[3], uint32_t *shadow);
+/* One sample with a per-vertex record (ORACLE_TRACE_STRIDE floats per
+ * surface vertex: depth, mesh, pdf, mean f|cos|/pdf, mean throughput at the
+ * vertex, mean NEE contribution, mean radiance after the vertex, trans);
+ * returns the number of vertices recorded (<= max_vertices). */
+#define ORACLE_TRACE_STRIDE 8
+uint32_t oracle_pt_trace_f32(const oracle_pt *pt, const oracle_pt_config *c, uint32_t px,
+                             uint32_t py, uint32_t frame, float rgb[3], float *trace,
+                             uint32_t max_vertices);
 /* Per-sample radiance image of one frame (W*H*3 floats). */
 void oracle_pt_render_frame_f32(const oracle_pt *pt, const oracle_pt_config *c,
                                 uint32_t frame, float *rgb, int nthreads,

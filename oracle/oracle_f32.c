@@ -171,6 +171,19 @@ void oracle_bvh_trace_any(const oracle_bvh *bvh, const oracle_ray *rays,
 }
 
 /* ---- integrator -------------------------------------------------------- */
+uint32_t oracle_pt_trace_f32(const oracle_pt *pt, const oracle_pt_config *c, uint32_t px,
+                             uint32_t py, uint32_t frame, float rgb[3], float *trace,
+                             uint32_t max_vertices)
+{
+    V3 r;
+    uint32_t n = 0;
+    pt_sample_traced(pt, c, px, py, frame, &r, NULL, trace, max_vertices, &n);
+    rgb[0] = r.x;
+    rgb[1] = r.y;
+    rgb[2] = r.z;
+    return n;
+}
+
 uint32_t oracle_pt_sample_f32(const oracle_pt *pt, const oracle_pt_config *c,
                               uint32_t px, uint32_t py, uint32_t frame,
                               float rgb[3], uint32_t *shadow)

@@ -96,6 +96,9 @@ def lib():
         L.oracle_pt_sample_f32.restype = C.c_uint32
         L.oracle_pt_sample_f32.argtypes = [C.c_void_p, C.POINTER(PTConfig), C.c_uint32,
                                            C.c_uint32, C.c_uint32, f32p, u32p]
+        L.oracle_pt_trace_f32.restype = C.c_uint32
+        L.oracle_pt_trace_f32.argtypes = [C.c_void_p, C.POINTER(PTConfig), C.c_uint32,
+                                          C.c_uint32, C.c_uint32, f32p, f32p, C.c_uint32]
         L.oracle_pt_render_frame_f32.argtypes = [C.c_void_p, C.POINTER(PTConfig),
                                                  C.c_uint32, f32p, C.c_int,
                                                  C.POINTER(C.c_uint64)]
@@ -309,6 +312,14 @@ class OraclePT:
         n = lib().oracle_pt_sample_f32(self.h, C.byref(cfg), x, y, frame,
                                        _p(rgb, f32p), C.byref(sh))
         return rgb, n, sh.value
+
+    def trace(self, cfg, x, y, frame, max_vertices=64):
+        """One sample with its per-vertex record (see oracle_pt_trace_f32)."""
+        rgb = np.zeros(3, np.float32)
+        tr = np.zeros((max_vertices, 8), np.float32)
+        n = lib().oracle_pt_trace_f32(self.h, C.byref(cfg), x, y, frame, _p(rgb, f32p),
+                                      _p(tr, f32p), max_vertices)
+        return rgb, tr[:n]
 
     def render_frame(self, cfg, frame, nthreads=None):
         out = np.zeros((cfg.height, cfg.width, 3), np.float32)

@@ -76,7 +76,8 @@ __device__ __forceinline__ float bias_at(const float *P, int i)
 }
 
 // ---- inference on query records -------------------------------------------------
-__global__ __launch_bounds__(kGuideBlock) void k_guide_infer(const bf16x8 *__restrict__ fwd,
+__global__ __launch_bounds__(kGuideBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_guide_infer(const bf16x8 *__restrict__ fwd,
                                                              const float *__restrict__ P,
                                                              const float *__restrict__ queries,
                                                              uint32_t n, float *__restrict__ out)

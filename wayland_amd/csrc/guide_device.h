@@ -131,11 +131,13 @@ __device__ __forceinline__ float vmf_log_norm(float kappa)
     return logf(kappa) - 1.83787706641f - logf(-expm1f(-2.0f * kappa));
 }
 
+// Decoded mixture: per lobe the log of (weight x vMF normaliser), the
+// concentration and the mean direction (5 floats, the registers a shade
+// kernel keeps across its BSDF and light samples).
 struct GuideMix
 {
-    float pi[kGuideLobes];
+    float lw[kGuideLobes];
     float kappa[kGuideLobes];
-    float lnorm[kGuideLobes];
     V3 mu[kGuideLobes];
 };
 
@@ -149,17 +151,13 @@ __device__ __forceinline__ void guide_decode(const float o[kGuideOut], GuideMix 
     float s = 0.f;
 #pragma unroll
     for (int j = 0; j < kGuideLobes; ++j)
-    {
-        m.pi[j] = expf(o[guide_row(j, 0)] - amax);
-        s += m.pi[j];
-    }
-    const float inv = 1.0f / s;
+        s += expf(o[guide_row(j, 0)] - amax);
+    const float lse = amax + logf(s);
 #pragma unroll
     for (int j = 0; j < kGuideLobes; ++j)
     {
-        m.pi[j] *= inv;
         m.kappa[j] = guide_softplus(o[guide_row(j, 1)]);
-        m.lnorm[j] = vmf_log_norm(m.kappa[j]);
+        m.lw[j] = (o[guide_row(j, 0)] - lse) + vmf_log_norm(m.kappa[j]);
         float st, ct, sp, cp;
         sincosf(o[guide_row(j, 2)], &st, &ct);
         sincosf(o[guide_row(j, 3)], &sp, &cp);
@@ -172,11 +170,12 @@ __device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w)
     float q = 0.f;
 #pragma unroll
     for (int j = 0; j < kGuideLobes; ++j)
-        q += m.pi[j] * expf(m.lnorm[j] + m.kappa[j] * (dot(m.mu[j], w) - 1.0f));
+        q += expf(m.lw[j] + m.kappa[j] * (dot(m.mu[j], w) - 1.0f));
     return q;
 }
 
-// u0 picks the lobe, (u1, u2) the direction (Jakob 2012 inversion).
+// u0 picks the lobe by the cumulative weights, (u1, u2) the direction
+// (Jakob 2012 inversion).
 __device__ __forceinline__ V3 guide_sample(const GuideMix &m, float u0, float u1, float u2)
 {
     float c = 0.f, kap = m.kappa[kGuideLobes - 1];
@@ -185,7 +184,7 @@ __device__ __forceinline__ V3 guide_sample(const GuideMix &m, float u0, float u1
 #pragma unroll
     for (int j = 0; j < kGuideLobes; ++j)
     {
-        c += m.pi[j];
+        c += expf(m.lw[j] - vmf_log_norm(m.kappa[j])); // lobe weight
         if (!found && u0 < c)
         {
             found = true;

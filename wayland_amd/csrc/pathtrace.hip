@@ -144,8 +144,8 @@ __device__ __forceinline__ void guide_query(const PtParams &P, V3 hitPos, V3 Ng,
     q[8] = -rayDir.z;
 }
 
-__global__ __launch_bounds__(kPtBlock) void k_guide_infer_pt(PtParams P, PtScene S,
-                                                             PtBuffers B, int parity)
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 {
     __shared__ bf16x8 sA[kGuideFwdVecs];
     __shared__ float sb[kGuideBiases];
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(kPtBlock) void k_guide_infer_pt(PtParams P, PtScene
 // MULTI: paths of many requests; each lane shades with its request's scene
 // and parameters.
 template <bool GUIDED, bool MULTI>
-__global__ __launch_bounds__(kPtBlock) void k_shade(PtParams P0, PtScene S0, PtBuffers B,
-                                                    int parity, const PtReq *__restrict__ reqs)
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3)))
+void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
 {
     const uint32_t count = B.counters[parity];
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
