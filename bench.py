@@ -48,7 +48,8 @@ def parse():
     p.add_argument("--guide-spp", type=int, default=64,
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
-    p.add_argument("--guide-ref-spp", type=int, default=2048)
+    p.add_argument("--guide-ref-spp", type=int, default=4096)
+    p.add_argument("--guide-runs", type=int, default=3)
     p.add_argument("--cornell-ref-spp", type=int, default=256,
                    help="FP64 oracle samples per pixel for the Cornell relMSE "
                         "(0 = skip the Cornell leg)")
@@ -131,28 +132,39 @@ def bench_guide(args, ctx, wl, torch):
         torch.cuda.synchronize()
         return pt.resolve(f, spp).cpu().numpy(), time.perf_counter() - t
 
-    guide = wl.Guide(ctx)
-    pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
-    img_g, t_g = progressive(args.guide_spp, 0)
-    ginfo = guide.info()
-    pt.set_guide(None)
-    # plain PT per-sample time, then the same wall time
-    _, t_probe = progressive(8, 1 << 20)
-    spp_p = max(1, int(round(t_g / (t_probe / 8))))
-    img_p, t_p = progressive(spp_p, 1 << 21)
-    e_g, e_p = relmse(img_g, ref), relmse(img_p, ref)
-    t_g1, t_p1 = relmse(img_g, ref, 1e-3), relmse(img_p, ref, 1e-3)
-    out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}, "
-                       "overhead view",
+    # independent runs: fresh network each time, training inside the timing
+    runs = args.guide_runs
+    g_rel, g_trim, g_t, steps = [], [], [], 0
+    for r in range(runs):
+        guide = wl.Guide(ctx, seed=r + 1)
+        pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
+        img_g, t_g = progressive(args.guide_spp, r << 20)
+        steps = guide.info()["steps"]
+        pt.set_guide(None)
+        guide.close()
+        g_rel.append(relmse(img_g, ref))
+        g_trim.append(relmse(img_g, ref, 1e-3))
+        g_t.append(t_g)
+    # plain PT per-sample time, then the same wall time as the guided runs
+    _, t_probe = progressive(8, 1 << 26)
+    spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 8))))
+    p_rel, p_trim, p_t = [], [], []
+    for r in range(runs):
+        img_p, t_p = progressive(spp_p, (1 << 27) + (r << 20))
+        p_rel.append(relmse(img_p, ref))
+        p_trim.append(relmse(img_p, ref, 1e-3))
+        p_t.append(t_p)
+    out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
            "reference": f"plain PT {ref_spp} spp on the GPU",
-           "guided": {"spp": args.guide_spp, "seconds": t_g, "relmse": e_g,
-                      "relmse_trim_0.1pct": t_g1, "adam_steps": ginfo["steps"]},
-           "plain": {"spp": spp_p, "seconds": t_p, "relmse": e_p,
-                     "relmse_trim_0.1pct": t_p1},
-           "relmse_ratio_equal_time": e_g / e_p,
-           "relmse_trim_ratio_equal_time": t_g1 / t_p1}
+           "runs": runs,
+           "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)),
+                      "relmse": g_rel, "relmse_trim_0.1pct": g_trim, "adam_steps": steps},
+           "plain": {"spp": spp_p, "seconds": float(np.mean(p_t)), "relmse": p_rel,
+                     "relmse_trim_0.1pct": p_trim},
+           "relmse_ratio_equal_time": float(np.mean(g_rel) / np.mean(p_rel)),
+           "relmse_median_ratio_equal_time": float(np.median(g_rel) / np.median(p_rel)),
+           "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim))}
     log(f"guide: {out}")
-    guide.close()
     return out
 
 

@@ -283,7 +283,8 @@ typedef struct wl_guide_config
     float learning_rate, beta1, beta2, epsilon;
     float mix_alpha;          /* probability of sampling the guide            */
     float max_weight;         /* training weights are clamped to this         */
-    uint32_t pad_[2];
+    uint32_t guided_bounces;  /* guide the first bounces only (0 = every one) */
+    uint32_t pad_;
 } wl_guide_config;
 
 typedef struct wl_guide_info

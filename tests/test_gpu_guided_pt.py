@@ -64,19 +64,19 @@ def _relmse(img, ref, trim=0.0):
 
 
 def test_guided_glossy_caustic_gain(gpu_ctx):
-    """Glossy caustic at 128 x 128: after training, guided PT has a lower
-    relMSE than plain PT at the same sample count against a converged plain
-    reference."""
+    """Glossy caustic at 256 x 256: guided PT (training included) has a
+    lower relMSE than plain PT at the same sample count against a converged
+    plain reference."""
     import torch
     import wayland_amd as wl
-    print("case: glossy caustic 128x128, guided vs plain", flush=True)
-    res, spp = 128, 64
+    print("case: glossy caustic 256x256, guided vs plain", flush=True)
+    res, spp = 256, 64
     sc = scenes.glossy_caustic(res, res)
     accel = wl.Accel(gpu_ctx, sc)
     pt = wl.PathTracer(gpu_ctx, accel, sc)
-    cfg_ref = wl.pt_config(res, res, spp=4096, frame_offset=1 << 22, max_depth=16)
+    cfg_ref = wl.pt_config(res, res, spp=8192, frame_offset=1 << 22, max_depth=16)
     film, _ = pt.render(cfg_ref, stats=False)
-    ref = pt.resolve(film, 4096).cpu().numpy()
+    ref = pt.resolve(film, 8192).cpu().numpy()
 
     def run(guided):
         cfg = wl.pt_config(res, res, spp=1, max_depth=16, max_batch_paths=res * res)
@@ -100,5 +100,5 @@ def test_guided_glossy_caustic_gain(gpu_ctx):
               f"({t_g:.3f} s), ratio {e_g / e_p:.3f}, "
               f"equal-time ratio {e_g * t_g / (e_p * t_p):.3f}")
     # the firefly-robust error: guided below plain at the same sample count
-    assert _relmse(img_g, ref, 0.001) < 0.8 * _relmse(img_p, ref, 0.001)
+    assert _relmse(img_g, ref, 0.001) < 0.7 * _relmse(img_p, ref, 0.001)
     guide.close()

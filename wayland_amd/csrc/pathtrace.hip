@@ -206,7 +206,7 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 }
 
 // ---- shade ------------------------------------------------------------------
-// GUIDED: one-sample MIS of the learned guide and the BSDF at vertices whose
+// GSAMPLE: one-sample MIS of the learned guide and the BSDF at vertices whose
 // material has trans = 0 and clearcoat = 0.  There the reference's pdf equals
 // the density its sampler draws from wherever the BSDF is non-zero, so
 // mixing in the guide keeps the estimator's expectation unchanged.  The mixed
@@ -214,7 +214,9 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 // weight, lastPdf for emitter-hit MIS, the NEE MIS weight).
 // MULTI: paths of many requests; each lane shades with its request's scene
 // and parameters.
-template <bool GUIDED, bool MULTI>
+// GTRAIN: paths carry guide training records (radiance after the recorded
+// vertex is summed; GSAMPLE vertices record themselves).
+template <bool GSAMPLE, bool GTRAIN, bool MULTI>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
 {
@@ -250,7 +252,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             const uint32_t prim = __float_as_uint(hit.y);
             bool done = false;
             // guiding: training record state of the path
-            const bool gtrain = GUIDED && (P.gmode & 2u);
+            const bool gtrain = GTRAIN && (P.gmode & 2u);
             float4 gacc = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gtrain)
                 gacc = B.gacc[path];
@@ -324,11 +326,11 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                             Ns = -Ns;
                     }
                     const bool guided =
-                        GUIDED && mat.trans == 0.0f && mat.clearcoat == 0.0f && (P.gmode & 1u);
+                        GSAMPLE && mat.trans == 0.0f && mat.clearcoat == 0.0f && (P.gmode & 1u);
                     GuideMix gm;
                     bool took_guide = false;
                     V3 nd;
-                    if (GUIDED && guided)
+                    if (GSAMPLE && guided)
                     {
                         float o[kGuideOut];
                         const float4 *gp = reinterpret_cast<const float4 *>(B.gout + 32 * (size_t)i);
@@ -357,7 +359,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                         nd = normalize(nd);
                     }
                     float pdf = pdf_disney(mat, Ns, -rayDir, nd);
-                    if (GUIDED && guided && !(pdf < 0.0f || __builtin_isnan(pdf)))
+                    if (GSAMPLE && guided && !(pdf < 0.0f || __builtin_isnan(pdf)))
                     {
                         pdf = P.galpha * guide_pdf(gm, nd) + (1.0f - P.galpha) * pdf;
                         if (!(pdf > 0.0f))
@@ -375,7 +377,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                         bsdf = clamp3(bsdf, 0.f, 1e30f);
                         lastTraceTerm = bsdf * fabsf(cosWeight) / pdf;
                         const V3 lastRayDir = rayDir;
-                        if (GUIDED && guided && gtrain && !grec_done)
+                        if (GSAMPLE && guided && gtrain && !grec_done)
                         {
                             // training record: query, sampled direction and
                             // the throughput reaching this vertex
@@ -443,7 +445,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                                     uptPdf = pdf_disney(mat, Ns, -lastRayDir, visRay);
                                     if (uptPdf < 0 || __builtin_isnan(uptPdf))
                                         uptPdf = 0.f;
-                                    if (GUIDED && guided)
+                                    if (GSAMPLE && guided)
                                         uptPdf = P.galpha * guide_pdf(gm, visRay) +
                                                  (1.0f - P.galpha) * uptPdf;
                                 }
@@ -824,16 +826,26 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, false>), dim3(g_trav),
                                dim3(kPtBlock), 0, st, S, B, parity, nullptr);
-        if (P.gmode)
+        // guided sampling on the first gbounces bounces (0 = all)
+        const bool gsample = (P.gmode & 1u) && (P.gbounces == 0 || b < P.gbounces);
+        const bool gtrain = (P.gmode & 2u) != 0;
+        if (gsample)
         {
             hipLaunchKernelGGL(k_guide_infer_pt, dim3(g_guide), dim3(kPtBlock), 0, st, P, S, B,
                                parity);
-            hipLaunchKernelGGL((k_shade<true, false>), dim3(g_shade), dim3(kPtBlock), 0, st, P,
-                               S, B, parity, nullptr);
+            if (gtrain)
+                hipLaunchKernelGGL((k_shade<true, true, false>), dim3(g_shade), dim3(kPtBlock), 0,
+                                   st, P, S, B, parity, nullptr);
+            else
+                hipLaunchKernelGGL((k_shade<true, false, false>), dim3(g_shade), dim3(kPtBlock),
+                                   0, st, P, S, B, parity, nullptr);
         }
+        else if (gtrain)
+            hipLaunchKernelGGL((k_shade<false, true, false>), dim3(g_shade), dim3(kPtBlock), 0, st,
+                               P, S, B, parity, nullptr);
         else
-            hipLaunchKernelGGL((k_shade<false, false>), dim3(g_shade), dim3(kPtBlock), 0, st, P,
-                               S, B, parity, nullptr);
+            hipLaunchKernelGGL((k_shade<false, false, false>), dim3(g_shade), dim3(kPtBlock), 0,
+                               st, P, S, B, parity, nullptr);
         if (shallow)
             hipLaunchKernelGGL((k_shadow<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
                                nullptr);
@@ -885,8 +897,8 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
                                dim3(kPtBlock), 0, st, S0, B, parity, L.reqs);
-        hipLaunchKernelGGL((k_shade<false, true>), dim3(g_shade), dim3(kPtBlock), 0, st, P0, S0,
-                           B, parity, L.reqs);
+        hipLaunchKernelGGL((k_shade<false, false, true>), dim3(g_shade), dim3(kPtBlock), 0, st,
+                           P0, S0, B, parity, L.reqs);
         if (L.stack16)
             hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, S0, B,
                                L.reqs);

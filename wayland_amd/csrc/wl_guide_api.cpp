@@ -89,14 +89,15 @@ wl_status wl_guide_default_config(wl_guide_config *c)
     std::memset(c, 0, sizeof *c);
     c->seed = 1;
     c->batch_size = 16384;
-    c->steps_per_batch = 4;
+    c->steps_per_batch = 8;
     c->max_samples = 1u << 20;
     c->learning_rate = 5e-3f;
     c->beta1 = 0.9f;
     c->beta2 = 0.99f;
     c->epsilon = 1e-8f;
     c->mix_alpha = 0.5f;
-    c->max_weight = 1e5f;
+    c->max_weight = 100.0f;
+    c->guided_bounces = 1;
     return WL_SUCCESS;
 }
 

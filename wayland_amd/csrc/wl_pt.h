@@ -58,6 +58,7 @@ struct PtParams
     // learned guiding (gmode 0 = off): bit 0 samples the guide at vertices
     // whose material is guidable, bit 1 also records training samples
     uint32_t gmode;
+    uint32_t gbounces;         // guided sampling on bounces [0, gbounces), 0 = all
     float galpha;              // probability of sampling the guide
     const void *gfwd;          // packed BF16 forward operands
     const float *gparams;      // FP32 master parameters (biases)

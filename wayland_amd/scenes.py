@@ -365,7 +365,8 @@ def wavefront_scene(count=25, subdiv=6, seed=4, width=1920, height=1080,
 def glossy_caustic(width=1024, height=1024, sphere_subdiv=6, floor_n=243) -> Scene:
     """Glossy-caustic workload: glass icosphere (IOR 1.5, Glass mapping) over a
     rough-metal floor grid (metallic 1, roughness 0.2) with a 0.1 x 0.1 quad
-    light behind the sphere, seen from above."""
+    light behind the sphere; the camera looks over the lit floor in front of
+    the sphere."""
     meshes = []
     sph = unit_icosphere_soup(sphere_subdiv) * 0.3 + np.array([0.0, 0.32, 0.0])
     meshes.append(soup_mesh(sph.astype(np.float32), glass_material(eta=1.5)))
@@ -385,8 +386,8 @@ def glossy_caustic(width=1024, height=1024, sphere_subdiv=6, floor_n=243) -> Sce
                        [h, 0.35 + 2 * h, -0.55 - h], [h, 0.35, -0.55 - h], [0, 0, 1])
     meshes.append(Mesh(lv, li, ln, matte_material((0, 0, 0)), light_L=(400.0,) * 3,
                        two_sided=False))
-    # overhead view of the floor: the light's cone on the metal floor with the
-    # sphere's shadow and the caustic the sphere focuses into it
-    cam = pinhole_camera([0.0, 4.0, 1.0], [0.0, -1.0, -1e-4], [0.0, 0.0, -1.0], 60.0,
+    # looking over the floor in front of the sphere: the light's cone on the
+    # metal floor with the sphere's shadow and the caustic focused into it
+    cam = pinhole_camera([0.0, 0.8, 3.5], [0.0, -0.8, -2.0], [0.0, 1.0, 0.0], 35.0,
                          width, height)
     return Scene(meshes, cam, (0.0, 0.0, 0.0), "glossy-caustic")
