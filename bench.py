@@ -50,6 +50,8 @@ def parse():
     p.add_argument("--guide-res", type=int, default=1024)
     p.add_argument("--guide-ref-spp", type=int, default=4096)
     p.add_argument("--guide-runs", type=int, default=3)
+    p.add_argument("--serve-requests", type=int, default=64,
+                   help="batched-serving requests (0 = skip)")
     p.add_argument("--cornell-ref-spp", type=int, default=256,
                    help="FP64 oracle samples per pixel for the Cornell relMSE "
                         "(0 = skip the Cornell leg)")
@@ -165,6 +167,58 @@ def bench_guide(args, ctx, wl, torch):
            "relmse_median_ratio_equal_time": float(np.median(g_rel) / np.median(p_rel)),
            "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim))}
     log(f"guide: {out}")
+    return out
+
+
+def bench_serve(args, ctx, wl, torch):
+    """Batched serving: 64 requests (seed 5) of Cornell and icosphere variants,
+    square resolution uniform in 128..1024, spp uniform in 1..64, depth
+    uniform in 8..16, through one server (megabatches replayed from a captured
+    hipGraph, capacity and shade grid autotuned on the workload).  Against the
+    same requests rendered one after another with wl_pt_render."""
+    from wayland_amd import scenes
+    rng = np.random.default_rng(5)
+    tracers = []
+    for k in range(4):
+        sc = scenes.cornell_box(256, 256, seed=k)
+        tracers.append(wl.PathTracer(ctx, wl.Accel(ctx, sc), sc))
+    for k in range(4):
+        sc = scenes.wavefront_scene(8, 4, seed=10 + k, width=256, height=256)
+        tracers.append(wl.PathTracer(ctx, wl.Accel(ctx, sc), sc))
+    reqs = []
+    for k in range(args.serve_requests):
+        res = int(rng.integers(128, 1025))
+        reqs.append({"pt": tracers[int(rng.integers(0, len(tracers)))], "width": res,
+                     "height": res, "spp": int(rng.integers(1, 65)),
+                     "max_depth": int(rng.integers(8, 17)), "frame_offset": k << 16})
+    paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+    server = wl.Server(ctx)
+    t0 = time.perf_counter()
+    tuned = server.autotune(reqs, max_spp=2)
+    t_tune = time.perf_counter() - t0
+    server.render(reqs)  # warm (graph capture)
+    films, st, done = server.render(reqs)
+    # the same requests one by one
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for r in reqs:
+        cfg = wl.pt_config(r["width"], r["height"], spp=r["spp"], max_depth=r["max_depth"],
+                           frame_offset=r["frame_offset"])
+        r["pt"].render(cfg, stats=False)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    done = np.array(done)
+    out = {"workload": f"{len(reqs)} requests, Cornell + icosphere variants, res 128..1024, "
+                       "spp 1..64, depth 8..16",
+           "paths": paths, "megabatches": st["batches"], "autotuned": tuned,
+           "autotune_seconds": t_tune,
+           "mpaths_per_s": paths / st["seconds"] / 1e6, "seconds": st["seconds"],
+           "grays_per_s": (st["extension_rays"] + st["shadow_rays"]) / st["seconds"] / 1e9,
+           "latency_p50_s": float(np.percentile(done, 50)),
+           "latency_p99_s": float(np.percentile(done, 99)),
+           "sequential_mpaths_per_s": paths / t_seq / 1e6, "sequential_seconds": t_seq}
+    log(f"serve: {out}")
+    server.close()
     return out
 
 
@@ -316,6 +370,10 @@ def main():
     if args.pt_spp > 0:
         pt_out = bench_pt(args, ctx, accel, sc, wl, torch)
 
+    serve_out = None
+    if args.serve_requests > 0:
+        serve_out = bench_serve(args, ctx, wl, torch)
+
     cornell_out = None
     if args.cornell_ref_spp > 0:
         cornell_out = bench_cornell(args, ctx, wl, torch)
@@ -359,6 +417,7 @@ def main():
         "pathtracer": pt_out,
         "guiding": guide_out,
         "cornell_cpu_workload": cornell_out,
+        "serving": serve_out,
     }
     print(json.dumps(out), flush=True)
 

@@ -76,7 +76,7 @@ __device__ __forceinline__ float bias_at(const float *P, int i)
 }
 
 // ---- inference on query records -------------------------------------------------
-__global__ __launch_bounds__(kGuideBlock) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(kGuideBlock)
 void k_guide_infer(const bf16x8 *__restrict__ fwd,
                                                              const float *__restrict__ P,
                                                              const float *__restrict__ queries,
@@ -128,7 +128,8 @@ __global__ void k_guide_mix_sample(const float *__restrict__ outs, const float *
             o[r] = outs[32 * (size_t)i + r];
         GuideMix m;
         guide_decode(o, m);
-        const V3 w = guide_sample(m, u[3 * (size_t)i], u[3 * (size_t)i + 1], u[3 * (size_t)i + 2]);
+        const V3 w = guide_sample(m, nullptr, u[3 * (size_t)i], u[3 * (size_t)i + 1],
+                                  u[3 * (size_t)i + 2]);
         dirs[3 * (size_t)i] = w.x;
         dirs[3 * (size_t)i + 1] = w.y;
         dirs[3 * (size_t)i + 2] = w.z;
@@ -178,16 +179,6 @@ __device__ __forceinline__ void batch_range(uint32_t n, uint32_t n_cap, uint32_t
 }
 
 // ---- training: forward, loss gradient, backward, staging ----------------------------
-__device__ __forceinline__ float wave_max4(float v)
-{
-    v = fmaxf(v, __shfl_xor(v, 16));
-    return fmaxf(v, __shfl_xor(v, 32));
-}
-__device__ __forceinline__ float wave_sum4(float v)
-{
-    v += __shfl_xor(v, 16);
-    return v + __shfl_xor(v, 32);
-}
 
 // d log-normaliser / d kappa = 1/kappa - 2 / (e^{2 kappa} - 1)
 __device__ __forceinline__ float vmf_dlog_norm(float kappa)

@@ -80,7 +80,9 @@ __device__ __forceinline__ void guide_forward(const bf16x8 *A, const float *bias
                                               bf16x8 x[2], floatx4 out[2], bf16x8 keep[5][2])
 {
     const int g = lane >> 4;
-#pragma unroll
+    // inference keeps one layer's operands live at a time; training unrolls
+    // the layers so the stored inputs stay in registers
+#pragma unroll(KEEP ? 4 : 1)
     for (int l = 0; l < 4; ++l)
     {
         floatx4 acc[4];
@@ -118,6 +120,18 @@ __device__ __forceinline__ void guide_forward(const bf16x8 *A, const float *bias
         for (int kb = 0; kb < 2; ++kb)
             out[t] = mfma16(A[guide_fwd_vec(4, t, kb) + lane], x[kb], out[t]);
     }
+}
+
+// ---- cross-lane reductions over the 4 lane groups (lanes l, l^16, l^32, l^48)
+__device__ __forceinline__ float wave_max4(float v)
+{
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+__device__ __forceinline__ float wave_sum4(float v)
+{
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
 }
 
 // ---- von Mises-Fisher mixture ------------------------------------------------
@@ -165,6 +179,53 @@ __device__ __forceinline__ void guide_decode(const float o[kGuideOut], GuideMix 
     }
 }
 
+// Decoded mixture as the path tracer stores it per query (48 floats):
+// float4 (lw, kappa, mu.x, mu.y) for lobes 0..7, then mu.z[8], then the lobe
+// weights pi[8].
+constexpr int kGuideDecoded = 48;
+
+// Decode of the outputs a lane holds after guide_forward (lobes 2 g and
+// 2 g + 1) into the stored form; every lane of the wave must call it.
+__device__ __forceinline__ void guide_decode_lanes(const floatx4 out[2], int g, float *dst,
+                                                   bool write)
+{
+    const float amax = wave_max4(fmaxf(out[0][0], out[1][0]));
+    const float se = wave_sum4(expf(out[0][0] - amax) + expf(out[1][0] - amax));
+    const float lse = amax + logf(se);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+    {
+        const int j = 2 * g + t;
+        const float kap = guide_softplus(out[t][1]);
+        const float lpi = out[t][0] - lse;
+        float st, ct, sp, cp;
+        sincosf(out[t][2], &st, &ct);
+        sincosf(out[t][3], &sp, &cp);
+        if (write)
+        {
+            reinterpret_cast<float4 *>(dst)[j] =
+                make_float4(lpi + vmf_log_norm(kap), kap, st * cp, st * sp);
+            dst[32 + j] = ct;
+            dst[40 + j] = expf(lpi);
+        }
+    }
+}
+
+__device__ __forceinline__ void guide_load(const float *src, GuideMix &m)
+{
+    const float4 *q = reinterpret_cast<const float4 *>(src);
+    const float4 z0 = q[8], z1 = q[9];
+    const float mz[8] = { z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w };
+#pragma unroll
+    for (int j = 0; j < kGuideLobes; ++j)
+    {
+        const float4 v = q[j];
+        m.lw[j] = v.x;
+        m.kappa[j] = v.y;
+        m.mu[j] = mk(v.z, v.w, mz[j]);
+    }
+}
+
 __device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w)
 {
     float q = 0.f;
@@ -174,9 +235,10 @@ __device__ __forceinline__ float guide_pdf(const GuideMix &m, V3 w)
     return q;
 }
 
-// u0 picks the lobe by the cumulative weights, (u1, u2) the direction
-// (Jakob 2012 inversion).
-__device__ __forceinline__ V3 guide_sample(const GuideMix &m, float u0, float u1, float u2)
+// u0 picks the lobe by the cumulative weights pi (read from memory, or
+// recomputed when pi is NULL), (u1, u2) the direction (Jakob 2012 inversion).
+__device__ __forceinline__ V3 guide_sample(const GuideMix &m, const float *pi, float u0,
+                                           float u1, float u2)
 {
     float c = 0.f, kap = m.kappa[kGuideLobes - 1];
     V3 mu = m.mu[kGuideLobes - 1];
@@ -184,7 +246,7 @@ __device__ __forceinline__ V3 guide_sample(const GuideMix &m, float u0, float u1
 #pragma unroll
     for (int j = 0; j < kGuideLobes; ++j)
     {
-        c += expf(m.lw[j] - vmf_log_norm(m.kappa[j])); // lobe weight
+        c += pi ? pi[j] : expf(m.lw[j] - vmf_log_norm(m.kappa[j])); // lobe weight
         if (!found && u0 < c)
         {
             found = true;

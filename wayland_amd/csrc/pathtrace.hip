@@ -144,7 +144,7 @@ __device__ __forceinline__ void guide_query(const PtParams &P, V3 hitPos, V3 Ng,
     q[8] = -rayDir.z;
 }
 
-__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(kPtBlock)
 void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 {
     __shared__ bf16x8 sA[kGuideFwdVecs];
@@ -195,13 +195,7 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
         guide_encode(q, g, x);
         floatx4 o[2];
         guide_forward<false>(sA, sb, lane, x, o, nullptr);
-        if (valid)
-        {
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-                *reinterpret_cast<float4 *>(B.gout + 32 * (size_t)i + 16 * t + 4 * g) =
-                    make_float4(o[t][0], o[t][1], o[t][2], o[t][3]);
-        }
+        guide_decode_lanes(o, g, B.gout + kGuideDecoded * (size_t)i, valid);
     }
 }
 
@@ -332,24 +326,14 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     V3 nd;
                     if (GSAMPLE && guided)
                     {
-                        float o[kGuideOut];
-                        const float4 *gp = reinterpret_cast<const float4 *>(B.gout + 32 * (size_t)i);
-#pragma unroll
-                        for (int k = 0; k < kGuideOut / 4; ++k)
-                        {
-                            const float4 v = gp[k];
-                            o[4 * k] = v.x;
-                            o[4 * k + 1] = v.y;
-                            o[4 * k + 2] = v.z;
-                            o[4 * k + 3] = v.w;
-                        }
-                        guide_decode(o, gm);
+                        const float *gsrc = B.gout + kGuideDecoded * (size_t)i;
+                        guide_load(gsrc, gm);
                         if (rnd(seed) < P.galpha)
                         {
                             const float u0 = rnd(seed);
                             const float u1 = rnd(seed);
                             const float u2 = rnd(seed);
-                            nd = guide_sample(gm, u0, u1, u2);
+                            nd = guide_sample(gm, gsrc + 40, u0, u1, u2);
                             took_guide = true;
                         }
                     }

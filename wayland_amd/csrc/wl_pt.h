@@ -82,7 +82,7 @@ struct PtBuffers
     uint32_t *counters;       // [0], [1] active queues, [2] shadow queue
     unsigned long long *stats; // [0] extension rays, [1] shadow rays
     // learned guiding (only with PtParams::gmode != 0)
-    float *gout;      // network outputs, 32 floats per queue slot
+    float *gout;      // decoded mixture, 48 floats per queue slot
     float4 *gacc;     // per path: radiance added after its first guided
                       // vertex; w = 1 once that vertex is recorded
     float *grec;      // per path: training record of that vertex (16 floats)

@@ -113,7 +113,7 @@ wl::PtBuffers carve(wl_scene s)
     {
         char *g = (char *)s->d_gbatch;
         B.gout = (float *)g;
-        g += (128 * cap + 255) & ~(size_t)255;
+        g += (192 * cap + 255) & ~(size_t)255;
         B.gacc = (float4 *)g;
         g += (16 * cap + 255) & ~(size_t)255;
         B.grec = (float *)g;
@@ -131,7 +131,7 @@ size_t batch_bytes(size_t cap) { return wl::pt_batch_bytes(cap); }
 size_t guide_batch_bytes(size_t cap)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(128 * cap) + r(16 * cap) + r(64 * cap);
+    return r(192 * cap) + r(16 * cap) + r(64 * cap);
 }
 
 wl_status ensure_capacity(wl_scene s, uint32_t paths)
