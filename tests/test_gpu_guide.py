@@ -139,6 +139,12 @@ def test_train_step_matches_oracle_adam(gpu_ctx):
     print(f"params off by > 0.1 lr: {(d > 0.1 * lr).mean():.4f}")
     assert (d > 0.1 * lr).mean() < 0.01
     assert info_steps(g) == 1
+    # the update kernel also rewrote the packed BF16 operands: inference now
+    # follows the new master weights
+    Q = random_samples(43, 200)[:, :9]
+    out = g.infer(_t(Q)).cpu().numpy()
+    ref = ob.guide_forward(P1, Q, emulate_bf16=True)
+    assert np.quantile(np.abs(out - ref) / (1.0 + np.abs(ref)), 0.999) < 2e-3
     g.close()
 
 

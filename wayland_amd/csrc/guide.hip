@@ -11,8 +11,9 @@
 //   dw       weight gradients dW_l = sum_s delta_s x_s^T as MFMA tiles with
 //            the samples as the K dimension, split over kGuideSplits slices
 //            (one partial sum per slice, no atomics);
-//   update   sums the slices, divides by the batch size, Adam;
-//   pack     FP32 master weights -> the BF16 operand layout of the kernels.
+//   update   sums the slices, divides by the batch size, Adam, and writes
+//            each new weight into its slots of the packed BF16 operands
+//            (k_guide_pack does the whole packing when weights are set).
 #include "guide_device.h"
 #include "wl_guide.h"
 
@@ -413,7 +414,28 @@ __global__ void k_guide_update(GuideDev G, const uint32_t *__restrict__ n_dev, u
         const float v = a.beta2 * G.adam_v[p] + (1.0f - a.beta2) * gr * gr;
         G.adam_m[p] = m;
         G.adam_v[p] = v;
-        G.params[p] -= a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+        const float w = G.params[p] - a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.eps);
+        G.params[p] = w;
+        // the weight's element in the packed operands (the inverse of k_guide_pack)
+        int l = 0;
+        while (l < 4 && p >= guide_w_off(l + 1))
+            ++l;
+        const int r = p - guide_w_off(l);
+        if (r < (l < 4 ? 64 : 32) * 64)
+        {
+            const int out = r / 64, in = r % 64;
+            const __bf16 wb = (__bf16)w;
+            const int fkb = (in / 16) / 2, fj = 4 * ((in / 16) % 2) + in % 4;
+            const int flane = (out % 16) + 16 * ((in % 16) / 4);
+            reinterpret_cast<__bf16 *>(G.fwd)[8 * (guide_fwd_vec(l, out / 16, fkb) + flane) + fj] = wb;
+            if (l >= 1)
+            {
+                const int bkb = (out / 16) / 2, bj = 4 * ((out / 16) % 2) + out % 4;
+                const int blane = (in % 16) + 16 * ((out % 16) / 4);
+                reinterpret_cast<__bf16 *>(G.bwd)[8 * (guide_bwd_vec(l, in / 16, bkb) + blane) + bj] =
+                    wb;
+            }
+        }
     }
 }
 
@@ -513,8 +535,6 @@ hipError_t guide_step(const GuideDev &G, const float *samples, const uint32_t *n
     if (loss)
         hipLaunchKernelGGL(k_guide_loss_mean, dim3(1), dim3(256), 0, s, G.loss, n_dev, n_cap,
                            step, batch, loss);
-    if (adam)
-        return guide_pack(G, s);
     return hipGetLastError();
 }
 

@@ -82,7 +82,7 @@ __device__ __forceinline__ void guide_forward(const bf16x8 *A, const float *bias
     const int g = lane >> 4;
     // inference keeps one layer's operands live at a time; training unrolls
     // the layers so the stored inputs stay in registers
-#pragma unroll(KEEP ? 4 : 1)
+#pragma unroll(KEEP ? 4 : 2)
     for (int l = 0; l < 4; ++l)
     {
         floatx4 acc[4];

@@ -82,15 +82,9 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
-// Request of a path in a multi-request megabatch (kept in rad.w).
-__device__ __forceinline__ uint32_t path_req(const PtBuffers &B, uint32_t path)
-{
-    return __float_as_uint(B.rad[path].w);
-}
-
 template <int STACK, bool MULTI>
-__global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int parity,
-                                                     const PtReq *__restrict__ reqs)
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(STACK == 16 ? 6 : 5)))
+void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[parity];
@@ -101,7 +95,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
         B.stats[0] += count;        // extension rays (single writer)
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
-    const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
+    const uint32_t *qreq = parity ? B.qreq1 : B.qreq0; // multi-request batches
     auto load = [&](uint32_t i, TravRay &r) {
         const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
         const float4 a = rp[0], b = rp[1];
@@ -115,7 +109,7 @@ __global__ __launch_bounds__(kPtBlock) void k_extend(PtScene S, PtBuffers B, int
         trav_persistent<false, kPtBlock, false, true>(
             nullptr, nullptr, count, stack + threadIdx.x,
             [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
-                const PtReq &R = reqs[path_req(B, qin[i])];
+                const PtReq &R = reqs[qreq[i]];
                 n = R.S.nodes;
                 t = R.S.tris;
                 load(i, r);
@@ -222,13 +216,15 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
     {
         const uint32_t i = base + threadIdx.x;
         bool cont = false, shadow = false, shadow_gacc = false;
-        uint32_t path = 0;
+        uint32_t path = 0, req = 0;
         wl_ray next_ray{}, shadow_ray{};
         V3 contrib = mk(0, 0, 0);
         if (i < count)
         {
             path = qin[i];
-            const PtReq *rq = MULTI ? reqs + path_req(B, path) : nullptr;
+            if (MULTI)
+                req = (parity ? B.qreq1 : B.qreq0)[i];
+            const PtReq *rq = MULTI ? reqs + req : nullptr;
             const PtParams &P = MULTI ? rq->P : P0;
             const PtScene &S = MULTI ? rq->S : S0;
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
@@ -457,7 +453,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     }
                 }
             }
-            B.rad[path] = f4(radiance, s_rad.w); // w: request id
+            B.rad[path] = f4(radiance, 0.0f);
             if (gtrain)
             {
                 // radiance reaching the path after its first guided vertex
@@ -490,11 +486,15 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
         {
             qout[slot] = path;
             rout[slot] = next_ray;
+            if (MULTI)
+                (parity ? B.qreq0 : B.qreq1)[slot] = req;
         }
         const uint32_t sslot = wave_append(&B.counters[2], shadow);
         if (shadow)
         {
             B.spath[sslot] = path;
+            if (MULTI)
+                B.sreq[sslot] = req;
             B.srays[sslot] = shadow_ray;
             B.scontrib[sslot] = f4(contrib, shadow_gacc ? 1.0f : 0.0f);
         }
@@ -503,8 +503,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
 
 // ---- shadow: any hit; unoccluded rays add their NEE contribution ----------
 template <int STACK, bool MULTI>
-__global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B,
-                                                     const PtReq *__restrict__ reqs)
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(STACK == 16 ? 6 : 5)))
+void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
     const uint32_t count = B.counters[2];
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(kPtBlock) void k_shadow(PtScene S, PtBuffers B,
         trav_persistent<true, kPtBlock, false, true>(
             nullptr, nullptr, count, stack + threadIdx.x,
             [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
-                const PtReq &R = reqs[path_req(B, B.spath[i])];
+                const PtReq &R = reqs[B.sreq[i]];
                 n = R.S.nodes;
                 t = R.S.tris;
                 load(i, r);
@@ -604,9 +604,10 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen_multi(PtMultiLaunch L, PtBu
         B.dir[p] = f4(d, __uint_as_float(seed));
         B.thr[p] = f4(mk(1, 1, 1), __uint_as_float(0u));
         B.ltt[p] = f4(mk(1, 1, 1), 0.0f);
-        B.rad[p] = f4(mk(0, 0, 0), __uint_as_float(it.req));
+        B.rad[p] = f4(mk(0, 0, 0), 0.0f);
         B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
         B.qpath0[p] = p;
+        B.qreq0[p] = it.req;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.counters[0] = npaths;

@@ -79,6 +79,8 @@ struct PtBuffers
     wl_ray *srays;
     uint32_t *spath;
     float4 *scontrib;
+    uint32_t *qreq0, *qreq1, *sreq; // multi-request batches: request of each
+                                    // ray queue slot / shadow queue slot
     uint32_t *counters;       // [0], [1] active queues, [2] shadow queue
     unsigned long long *stats; // [0] extension rays, [1] shadow rays
     // learned guiding (only with PtParams::gmode != 0)
@@ -96,8 +98,8 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                         int num_cus, hipStream_t st);
 
 // ---- batched serving: one megabatch mixes paths of many requests ----------
-// A request's scene, parameters and film; a path finds its request through
-// the id kept in rad.w.
+// A request's scene, parameters and film; the ray and shadow queues of a
+// multi-request batch carry each entry's request id (qreq0/qreq1/sreq).
 struct PtReq
 {
     PtParams P; // pix0 / npix / nsamples / frame0 unused; gmode 0

@@ -42,6 +42,9 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
     B.srays = (wl_ray *)take(32 * cap);
     B.spath = (uint32_t *)take(4 * cap);
     B.scontrib = (float4 *)take(16 * cap);
+    B.qreq0 = (uint32_t *)take(4 * cap);
+    B.qreq1 = (uint32_t *)take(4 * cap);
+    B.sreq = (uint32_t *)take(4 * cap);
     B.counters = counters;
     B.stats = stats;
     return B;
@@ -50,7 +53,7 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
 size_t pt_batch_bytes(size_t cap)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 3 + r(16 * cap) * 2;
+    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 6 + r(16 * cap) * 2;
 }
 
 PtScene pt_scene_view(wl_scene s)
