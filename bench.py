@@ -185,40 +185,50 @@ def bench_serve(args, ctx, wl, torch):
     for k in range(4):
         sc = scenes.wavefront_scene(8, 4, seed=10 + k, width=256, height=256)
         tracers.append(wl.PathTracer(ctx, wl.Accel(ctx, sc), sc))
-    reqs = []
-    for k in range(args.serve_requests):
-        res = int(rng.integers(128, 1025))
-        reqs.append({"pt": tracers[int(rng.integers(0, len(tracers)))], "width": res,
-                     "height": res, "spp": int(rng.integers(1, 65)),
-                     "max_depth": int(rng.integers(8, 17)), "frame_offset": k << 16})
-    paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
-    server = wl.Server(ctx)
-    t0 = time.perf_counter()
-    tuned = server.autotune(reqs, max_spp=2)
-    t_tune = time.perf_counter() - t0
-    server.render(reqs)  # warm (graph capture)
-    films, st, done = server.render(reqs)
-    # the same requests one by one
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for r in reqs:
-        cfg = wl.pt_config(r["width"], r["height"], spp=r["spp"], max_depth=r["max_depth"],
-                           frame_offset=r["frame_offset"])
-        r["pt"].render(cfg, stats=False)
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t
-    done = np.array(done)
-    out = {"workload": f"{len(reqs)} requests, Cornell + icosphere variants, res 128..1024, "
-                       "spp 1..64, depth 8..16",
-           "paths": paths, "megabatches": st["batches"], "autotuned": tuned,
-           "autotune_seconds": t_tune,
-           "mpaths_per_s": paths / st["seconds"] / 1e6, "seconds": st["seconds"],
-           "grays_per_s": (st["extension_rays"] + st["shadow_rays"]) / st["seconds"] / 1e9,
-           "latency_p50_s": float(np.percentile(done, 50)),
-           "latency_p99_s": float(np.percentile(done, 99)),
-           "sequential_mpaths_per_s": paths / t_seq / 1e6, "sequential_seconds": t_seq}
-    log(f"serve: {out}")
-    server.close()
+    def workload(n, res_lo, res_hi, spp_hi, seed_shift):
+        reqs = []
+        for k in range(n):
+            res = int(rng.integers(res_lo, res_hi + 1))
+            reqs.append({"pt": tracers[int(rng.integers(0, len(tracers)))], "width": res,
+                         "height": res, "spp": int(rng.integers(1, spp_hi + 1)),
+                         "max_depth": int(rng.integers(8, 17)),
+                         "frame_offset": (k + seed_shift) << 16})
+        return reqs
+
+    def run(reqs, label):
+        paths = sum(r["width"] * r["height"] * r["spp"] for r in reqs)
+        server = wl.Server(ctx)
+        t0 = time.perf_counter()
+        tuned = server.autotune(reqs, max_spp=2)
+        t_tune = time.perf_counter() - t0
+        server.render(reqs)  # warm (graph capture)
+        films, st, done = server.render(reqs)
+        server.close()
+        # the same requests one by one (launches queued back to back)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for r in reqs:
+            cfg = wl.pt_config(r["width"], r["height"], spp=r["spp"], max_depth=r["max_depth"],
+                               frame_offset=r["frame_offset"])
+            r["pt"].render(cfg, stats=False)
+        torch.cuda.synchronize()
+        t_seq = time.perf_counter() - t
+        done = np.array(done)
+        out = {"workload": label, "paths": paths, "megabatches": st["batches"],
+               "autotuned": tuned, "autotune_seconds": t_tune,
+               "mpaths_per_s": paths / st["seconds"] / 1e6, "seconds": st["seconds"],
+               "grays_per_s": (st["extension_rays"] + st["shadow_rays"]) / st["seconds"] / 1e9,
+               "latency_p50_s": float(np.percentile(done, 50)),
+               "latency_p99_s": float(np.percentile(done, 99)),
+               "sequential_mpaths_per_s": paths / t_seq / 1e6, "sequential_seconds": t_seq}
+        log(f"serve: {out}")
+        return out
+
+    out = run(workload(args.serve_requests, 128, 1024, 64, 0),
+              f"{args.serve_requests} requests, Cornell + icosphere variants, res 128..1024, "
+              "spp 1..64, depth 8..16")
+    out["small_requests"] = run(workload(256, 64, 256, 4, 1 << 10),
+                                "256 requests, res 64..256, spp 1..4, depth 8..16")
     return out
 
 
