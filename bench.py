@@ -84,11 +84,21 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
                                    frame_offset=10_000), None, stats=True)
     rays_per_path = (st["extension_rays"] + st["shadow_rays"]) / st["paths"]
     paths = W * H * args.pt_spp
+    # the same render with the material-sorted shade queue
+    film.zero_()
+    cfg_s = wl.pt_config(W, H, spp=args.pt_spp, max_depth=16, rr_depth=4,
+                         flags=wl.PT_SORT_BY_MATERIAL)
+    start.record()
+    pt.render(cfg_s, film, stats=False)
+    end.record()
+    torch.cuda.synchronize()
+    dt_sorted = start.elapsed_time(end) / 1e3
     out = {"workload": f"wavefront PT {W}x{H} {args.pt_spp} spp depth 16 RR>4, "
                        f"{wsc.num_triangles} tris, 2 quad lights + env 0.1",
            "mpaths_per_s": paths / dt / 1e6,
            "grays_per_s": paths * rays_per_path / dt / 1e9,
-           "rays_per_path": rays_per_path, "seconds": dt}
+           "rays_per_path": rays_per_path, "seconds": dt,
+           "mpaths_per_s_material_sorted": paths / dt_sorted / 1e6}
     log(f"pt: {out}")
     return out
 

@@ -29,6 +29,7 @@ HIT_DTYPE = np.dtype([("t", np.float32), ("prim", np.uint32),
 
 STRATEGY_UPT, STRATEGY_NEE, STRATEGY_MIS = 0, 1, 2
 PT_SORT_BY_MATERIAL = 1
+PT_SORT_BY_MATERIAL = 1
 
 
 def _torch():

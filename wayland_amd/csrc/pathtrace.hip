@@ -79,6 +79,9 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.counters[0] = npaths;
+    if (blockIdx.x == 0 && P.sort_bins)
+        for (uint32_t b = threadIdx.x; b < P.sort_bins; b += kPtBlock)
+            B.sort_bins[b] = 0;
 }
 
 // ---- extend: closest hit for the active queue -----------------------------
@@ -118,6 +121,105 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     else
         trav_persistent<false, kPtBlock, false>(S.nodes, S.tris, count, stack + threadIdx.x,
                                                 load, store);
+}
+
+// ---- material sort of the shade queue ------------------------------------------
+// Counting sort of the queue entries by the mesh (= material slot) they hit;
+// misses take the last bin.  Per block: an LDS histogram, one global atomic per
+// bin; the scatter reserves each block's range per bin the same way, so the
+// global atomics per bounce are (blocks x bins), not one per entry.
+constexpr uint32_t kSortMaxBins = 1024;
+
+__device__ __forceinline__ uint32_t sort_key(const PtScene &S, uint32_t prim, uint32_t nbins)
+{
+    if (prim == WL_INVALID)
+        return nbins - 1;
+    const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
+    return min(__float_as_uint(sr[2].y), nbins - 2);
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_sort_count(PtParams P, PtScene S, PtBuffers B,
+                                                         int parity)
+{
+    __shared__ uint32_t hist[kSortMaxBins];
+    const uint32_t nb = P.sort_bins;
+    for (uint32_t b = threadIdx.x; b < nb; b += kPtBlock)
+        hist[b] = 0;
+    __syncthreads();
+    const uint32_t count = B.counters[parity];
+    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < count; i += gridDim.x * kPtBlock)
+    {
+        const uint32_t key =
+            sort_key(S, __float_as_uint(reinterpret_cast<const float4 *>(B.hits)[i].y), nb);
+        B.sort_keys[i] = key;
+        atomicAdd(&hist[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += kPtBlock)
+        if (hist[b])
+            atomicAdd(&B.sort_bins[b], hist[b]);
+}
+
+// One block: bin counts -> running bin offsets; clears the counts.
+__global__ void k_sort_scan(PtParams P, PtBuffers B)
+{
+    __shared__ uint32_t part[kPtBlock];
+    const uint32_t nb = P.sort_bins;
+    const uint32_t per = (nb + kPtBlock - 1) / kPtBlock;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    uint32_t s = 0;
+    for (uint32_t b = b0; b < b1; ++b)
+        s += B.sort_bins[b];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t run = 0;
+        for (int t = 0; t < kPtBlock; ++t)
+        {
+            const uint32_t v = part[t];
+            part[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (uint32_t b = b0; b < b1; ++b)
+    {
+        const uint32_t c = B.sort_bins[b];
+        B.sort_bins[kSortMaxBins + b] = run;
+        B.sort_bins[b] = 0;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(kPtBlock) void k_sort_scatter(PtParams P, PtBuffers B, int parity)
+{
+    __shared__ uint32_t hist[kSortMaxBins];
+    __shared__ uint32_t base[kSortMaxBins];
+    const uint32_t nb = P.sort_bins;
+    const uint32_t count = B.counters[parity];
+    for (uint32_t start = blockIdx.x * kPtBlock; start < count; start += gridDim.x * kPtBlock)
+    {
+        for (uint32_t b = threadIdx.x; b < nb; b += kPtBlock)
+            hist[b] = 0;
+        __syncthreads();
+        const uint32_t i = start + threadIdx.x;
+        uint32_t key = 0, local = 0;
+        if (i < count)
+        {
+            key = B.sort_keys[i];
+            local = atomicAdd(&hist[key], 1u);
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < nb; b += kPtBlock)
+            if (hist[b])
+                base[b] = atomicAdd(&B.sort_bins[kSortMaxBins + b], hist[b]);
+        __syncthreads();
+        if (i < count)
+            B.sort_perm[base[key] + local] = i;
+        __syncthreads();
+    }
 }
 
 // ---- guide inference for the active queue ------------------------------------
@@ -214,12 +316,15 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
     wl_ray *rout = parity ? B.rays0 : B.rays1;
     for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
     {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t j = base + threadIdx.x;
+        const bool in_range = j < count;
+        // material-sorted order: the j-th entry of the sorted queue
+        const uint32_t i = (!MULTI && P0.sort_bins && in_range) ? B.sort_perm[j] : j;
         bool cont = false, shadow = false, shadow_gacc = false;
         uint32_t path = 0, req = 0;
         wl_ray next_ray{}, shadow_ray{};
         V3 contrib = mk(0, 0, 0);
-        if (i < count)
+        if (in_range)
         {
             path = qin[i];
             if (MULTI)
@@ -798,6 +903,7 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     const uint32_t g_trav =
         std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (uint32_t)(shallow ? occ16 : occ32));
     const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
+    const uint32_t g_sort = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 4);
     // guide inference: 64 queries per workgroup pass, up to 4 workgroups per CU
     const uint32_t g_guide =
         std::max<uint32_t>(1, std::min<uint32_t>((npaths + 63) / 64, (uint32_t)num_cus * 4));
@@ -811,6 +917,12 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, false>), dim3(g_trav),
                                dim3(kPtBlock), 0, st, S, B, parity, nullptr);
+        if (P.sort_bins)
+        {
+            hipLaunchKernelGGL(k_sort_count, dim3(g_sort), dim3(kPtBlock), 0, st, P, S, B, parity);
+            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(kPtBlock), 0, st, P, B);
+            hipLaunchKernelGGL(k_sort_scatter, dim3(g_sort), dim3(kPtBlock), 0, st, P, B, parity);
+        }
         // guided sampling on the first gbounces bounces (0 = all)
         const bool gsample = (P.gmode & 1u) && (P.gbounces == 0 || b < P.gbounces);
         const bool gtrain = (P.gmode & 2u) != 0;

@@ -57,6 +57,7 @@ struct PtParams
     wl_pinhole_camera cam;
     // learned guiding (gmode 0 = off): bit 0 samples the guide at vertices
     // whose material is guidable, bit 1 also records training samples
+    uint32_t sort_bins;        // material-sorted shade queue: bins (0 = off)
     uint32_t gmode;
     uint32_t gbounces;         // guided sampling on bounces [0, gbounces), 0 = all
     float galpha;              // probability of sampling the guide
@@ -79,6 +80,8 @@ struct PtBuffers
     wl_ray *srays;
     uint32_t *spath;
     float4 *scontrib;
+    uint32_t *sort_keys, *sort_perm; // material sort: key and order per queue slot
+    uint32_t *sort_bins;             // [0, 1024) counts, [1024, 2048) offsets
     uint32_t *qreq0, *qreq1, *sreq; // multi-request batches: request of each
                                     // ray queue slot / shadow queue slot
     uint32_t *counters;       // [0], [1] active queues, [2] shadow queue

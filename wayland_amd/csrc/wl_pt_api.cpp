@@ -42,6 +42,9 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
     B.srays = (wl_ray *)take(32 * cap);
     B.spath = (uint32_t *)take(4 * cap);
     B.scontrib = (float4 *)take(16 * cap);
+    B.sort_keys = (uint32_t *)take(4 * cap);
+    B.sort_perm = (uint32_t *)take(4 * cap);
+    B.sort_bins = (uint32_t *)take(4 * 2048);
     B.qreq0 = (uint32_t *)take(4 * cap);
     B.qreq1 = (uint32_t *)take(4 * cap);
     B.sreq = (uint32_t *)take(4 * cap);
@@ -53,7 +56,7 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
 size_t pt_batch_bytes(size_t cap)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 6 + r(16 * cap) * 2;
+    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 8 + r(16 * cap) * 2 + r(4 * 2048);
 }
 
 PtScene pt_scene_view(wl_scene s)
@@ -189,6 +192,8 @@ wl_status check_config(wl_scene s, const wl_pt_config *c) { return wl::pt_check_
 wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
 {
     wl::PtParams P = wl::pt_params_for(s, c);
+    if (c->flags & WL_PT_SORT_BY_MATERIAL)
+        P.sort_bins = std::min<uint32_t>(s->num_meshes + 1, 1024);
     if (s->guide && s->guide_mode)
     {
         P.gmode = s->guide_mode;
