@@ -213,7 +213,7 @@ typedef struct wl_pt_config
     float epsilon;         /* PathTracing.cu:19 (1e-3)                         */
     uint32_t strategy;     /* PathTracing.cu:21-29 (MIS)                       */
     uint32_t flags;
-    uint32_t max_batch_paths; /* 0 = library default                          */
+    uint32_t max_batch_paths; /* 0 = library default (2^26)                   */
     uint32_t pad_;
 } wl_pt_config;
 

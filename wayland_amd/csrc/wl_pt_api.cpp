@@ -207,7 +207,9 @@ wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
     return P;
 }
 
-constexpr uint32_t kDefaultBatchPaths = 1u << 23;
+// Paths per batch when the caller gives none: 2^26 paths hold 16 GB of path
+// state and queues, so the later, shorter bounces still fill the GPU.
+constexpr uint32_t kDefaultBatchPaths = 1u << 26;
 
 } // namespace
 
