@@ -69,11 +69,12 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
     waccel = wl.Accel(ctx, wsc)
     pt = wl.PathTracer(ctx, waccel, wsc)
     film = torch.zeros((W * H, 6), dtype=torch.float32, device="cuda")
-    warm = wl.pt_config(W, H, spp=1, max_depth=16, rr_depth=4)
-    pt.render(warm, film)
+    # 2^27 paths per batch (32 GB of path state): 1080p x 64 spp in one batch
+    cfg = wl.pt_config(W, H, spp=args.pt_spp, max_depth=16, rr_depth=4,
+                       max_batch_paths=1 << 27)
+    pt.render(cfg, film, stats=False)  # warm: allocates the batch buffers
     torch.cuda.synchronize()
     film.zero_()
-    cfg = wl.pt_config(W, H, spp=args.pt_spp, max_depth=16, rr_depth=4)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record()
     _, st = pt.render(cfg, film, stats=False)
@@ -87,12 +88,14 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
     # the same render with the material-sorted shade queue
     film.zero_()
     cfg_s = wl.pt_config(W, H, spp=args.pt_spp, max_depth=16, rr_depth=4,
-                         flags=wl.PT_SORT_BY_MATERIAL)
+                         max_batch_paths=1 << 27, flags=wl.PT_SORT_BY_MATERIAL)
     start.record()
     pt.render(cfg_s, film, stats=False)
     end.record()
     torch.cuda.synchronize()
     dt_sorted = start.elapsed_time(end) / 1e3
+    pt.close()
+    waccel.close()
     out = {"workload": f"wavefront PT {W}x{H} {args.pt_spp} spp depth 16 RR>4, "
                        f"{wsc.num_triangles} tris, 2 quad lights + env 0.1",
            "mpaths_per_s": paths / dt / 1e6,
