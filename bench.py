@@ -147,6 +147,12 @@ def bench_guide(args, ctx, wl, torch):
         torch.cuda.synchronize()
         return pt.resolve(f, spp).cpu().numpy(), time.perf_counter() - t
 
+    # allocate the per-path guide buffers outside the timed runs
+    warm_guide = wl.Guide(ctx)
+    pt.set_guide(warm_guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
+    progressive(1, 1 << 28)
+    pt.set_guide(None)
+    warm_guide.close()
     # independent runs: fresh network each time, training inside the timing
     runs = args.guide_runs
     g_rel, g_trim, g_t, steps = [], [], [], 0
