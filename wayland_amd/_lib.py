@@ -219,6 +219,11 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                           "or python wayland_amd/build.py")
+    # torch ships its own HIP runtime under the same soname.  Loading torch
+    # first makes the library bind to that one runtime; loaded the other way
+    # round, the process holds two runtimes and torch then sees no device
+    # (and tensor pointers would belong to the other runtime).
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name, None)
