@@ -190,6 +190,37 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
 wl_status wl_scene_destroy(wl_scene scene);
 wl_status wl_scene_set_camera(wl_scene scene, const wl_pinhole_camera *camera);
 
+/* An RGBA8 image, rows top to bottom: what stbi_load(path, ..., 4) returns
+ * in Texture::TexLoadImage (src/Frontend/Core/Texture.cpp:13-16). */
+typedef struct wl_texture_image
+{
+    const uint8_t *rgba;        /* host, 4 * width * height bytes              */
+    uint32_t width, height;
+} wl_texture_image;
+
+/* Texture binding of one build input (the hasTexture / texture / texcoords
+ * fields GenerateSBT fills, PathTracingProgramManager.cpp:97-119). */
+typedef struct wl_mesh_texture
+{
+    const float *texcoords;     /* host, 2 floats per vertex, or NULL: all zero,
+                                   as the reference's zero-filled texcoord
+                                   buffer gives a mesh without uv            */
+    uint32_t texture;           /* image index, or WL_INVALID_INDEX            */
+    uint32_t pad_;
+} wl_mesh_texture;
+
+/* Textures of a scene: UploadTextures (DeviceManager.cpp:183-229) plus the
+ * per-mesh binding.  At a hit on a textured mesh the material's base colour
+ * is multiplied by the filtered texel (PathTracing.cu:261-268, Evaluate.h:21,
+ * 86), for the bounce and for the NEE at that vertex (PathTracing.cu:120-122,
+ * 280).  Sampling follows the reference's texture object: normalized
+ * coordinates, wrap addressing, bilinear filtering with 8-bit fractional
+ * weights, texels read as u8 / 255.  One entry per build input;
+ * num_images == 0 removes the scene's textures. */
+wl_status wl_scene_set_textures(wl_scene scene, const wl_texture_image *images,
+                                uint32_t num_images, const wl_mesh_texture *meshes,
+                                uint32_t num_meshes);
+
 /* ---- wavefront path tracer (PathTracing.cu:40-291) --------------------- */
 enum
 {

@@ -120,6 +120,10 @@ void oracle_disney_sample_f64(const oracle_material *m, const double Ns[3],
                               uint32_t *seed, double L[3]);
 /* Orthonormal basis, Common.h:102-116 (Duff et al. 2017). */
 void oracle_onb_f32(const float n[3], float u[3], float v[3]);
+/* Texture fetch of the reference's texture object (wrap, bilinear with 8-bit
+ * fractional weights, u8 / 255) on an RGBA8 image, rows top to bottom. */
+void oracle_tex_bilinear_f32(const uint8_t *rgba, uint32_t w, uint32_t h, float u, float v,
+                             float out[3]);
 /* ACES fit tonemap, Tonemap.h:34-42. */
 void oracle_aces_f32(const float rad[3], float out[3]);
 
@@ -142,6 +146,13 @@ typedef struct
     const uint32_t *light_two_sided;
     float cam_pos[3], cam_lookat[3], cam_up[3], cam_right[3];
     float bg[3];                    /* MissData::bg_color                    */
+    /* textures (PathTracing.cu:261-268); num_textures 0 = none             */
+    uint32_t num_textures;
+    const float *texcoords;         /* float2 per vertex, indexed as vertices */
+    const uint32_t *mesh_texture;   /* image per mesh or ORACLE_INVALID      */
+    const uint32_t *texture_offset; /* first texel of each image             */
+    const uint32_t *texture_size;   /* width, height per image               */
+    const uint8_t *texels;          /* RGBA8, rows top to bottom             */
 } oracle_scene_desc;
 
 typedef struct

@@ -395,6 +395,23 @@ oracle_pt *oracle_pt_create(const oracle_scene_desc *d)
     memcpy(pt->cam_up, d->cam_up, 12);
     memcpy(pt->cam_right, d->cam_right, 12);
     memcpy(pt->bg, d->bg, 12);
+    if (d->num_textures)
+    {
+        uint64_t ntex = 0;
+        for (uint32_t i = 0; i < d->num_textures; ++i)
+        {
+            uint64_t end = (uint64_t)d->texture_offset[i] +
+                           (uint64_t)d->texture_size[2 * i] * d->texture_size[2 * i + 1];
+            if (end > ntex)
+                ntex = end;
+        }
+        pt->num_textures = d->num_textures;
+        pt->texcoords = (float *)dupmem(d->texcoords, sizeof(float) * 2 * (size_t)nv);
+        pt->mesh_texture = (uint32_t *)dupmem(d->mesh_texture, 4 * (size_t)M);
+        pt->texture_offset = (uint32_t *)dupmem(d->texture_offset, 4 * (size_t)d->num_textures);
+        pt->texture_size = (uint32_t *)dupmem(d->texture_size, 8 * (size_t)d->num_textures);
+        pt->texels = (uint8_t *)dupmem(d->texels, 4 * (size_t)ntex);
+    }
     /* gather world-space triangles in global primitive order */
     float *tris = (float *)malloc(sizeof(float) * 9 * (size_t)(nt ? nt : 1));
     for (uint32_t m = 0; m < M; ++m)
@@ -428,6 +445,11 @@ void oracle_pt_free(oracle_pt *pt)
     free(pt->light_mesh);
     free(pt->light_L);
     free(pt->light_two_sided);
+    free(pt->texcoords);
+    free(pt->mesh_texture);
+    free(pt->texture_offset);
+    free(pt->texture_size);
+    free(pt->texels);
     oracle_bvh_free(pt->bvh);
     free(pt);
 }

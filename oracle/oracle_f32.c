@@ -20,6 +20,7 @@
 #define RFMAX fmaxf
 #define RISNAN isnan
 #define RISINF isinf
+#define RFLOOR floorf
 
 #include "oracle_impl.inc"
 
@@ -358,4 +359,13 @@ void oracle_example_render(const oracle_pt *pt, const float *mesh_color,
     ec.row0 = row0;
     ec.rgba = rgba;
     oracle_parallel_for((row1 - row0) * c->height, 64, nthreads, ex_task, &ec);
+}
+
+void oracle_tex_bilinear_f32(const uint8_t *rgba, uint32_t w, uint32_t h, float u, float v,
+                             float out[3])
+{
+    V3 r = tex_bilinear_rgba(rgba, w, h, u, v);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
 }

@@ -20,6 +20,7 @@
 #define RFMAX fmax
 #define RISNAN isnan
 #define RISINF isinf
+#define RFLOOR floor
 
 #include "oracle_impl.inc"
 

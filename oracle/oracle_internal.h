@@ -45,6 +45,12 @@ struct oracle_pt
     uint32_t *light_two_sided;
     float cam_pos[3], cam_lookat[3], cam_up[3], cam_right[3];
     float bg[3];
+    uint32_t num_textures;
+    float *texcoords;
+    uint32_t *mesh_texture;
+    uint32_t *texture_offset;
+    uint32_t *texture_size;
+    uint8_t *texels;
     oracle_bvh *bvh;
 };
 

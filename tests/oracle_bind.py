@@ -33,6 +33,9 @@ class SceneDesc(C.Structure):
         ("cam_pos", C.c_float * 3), ("cam_lookat", C.c_float * 3),
         ("cam_up", C.c_float * 3), ("cam_right", C.c_float * 3),
         ("bg", C.c_float * 3),
+        ("num_textures", C.c_uint32),
+        ("texcoords", f32p), ("mesh_texture", u32p), ("texture_offset", u32p),
+        ("texture_size", u32p), ("texels", C.c_void_p),
     ]
 
 
@@ -90,6 +93,8 @@ def lib():
             getattr(L, f"oracle_disney_sample_{sfx}").argtypes = [C.c_void_p, fp, fp, fp, u32p, fp]
         L.oracle_onb_f32.argtypes = [f32p, f32p, f32p]
         L.oracle_aces_f32.argtypes = [f32p, f32p]
+        L.oracle_tex_bilinear_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
+                                              C.c_float, f32p]
         L.oracle_pt_create.restype = C.c_void_p
         L.oracle_pt_create.argtypes = [C.POINTER(SceneDesc)]
         L.oracle_pt_free.argtypes = [C.c_void_p]
@@ -273,6 +278,15 @@ def aces(rad):
     return out
 
 
+def tex_bilinear(rgba, u, v):
+    """Filtered texel of an (H, W, 4) uint8 image at normalized (u, v)."""
+    img = np.ascontiguousarray(rgba, np.uint8)
+    out = np.zeros(3, np.float32)
+    lib().oracle_tex_bilinear_f32(img.ctypes.data, img.shape[1], img.shape[0], u, v,
+                                  _p(out, f32p))
+    return out
+
+
 # ---- path tracer ------------------------------------------------------------
 class OraclePT:
     """The oracle's scene + integrator (PathTracing.cu restated)."""
@@ -299,6 +313,13 @@ class OraclePT:
             d.cam_up[:] = list(cam.up)
             d.cam_right[:] = list(cam.right)
         d.bg[:] = list(f["bg"])
+        d.num_textures = f["texture_offset"].shape[0]
+        if d.num_textures:
+            d.texcoords = _p(f["texcoords"], f32p)
+            d.mesh_texture = _p(f["mesh_texture"], u32p)
+            d.texture_offset = _p(f["texture_offset"], u32p)
+            d.texture_size = _p(f["texture_size"], u32p)
+            d.texels = f["texels"].ctypes.data
         self.h = lib().oracle_pt_create(C.byref(d))
 
     @staticmethod

@@ -271,8 +271,12 @@ WorldEnd
 @pytest.mark.parametrize("snippet,exc", [
     ('Material "matte"\nShape "sphere" "float radius" [1]', NotImplementedError),
     ('Material "metal"', NotImplementedError),
-    ('Texture "t" "color" "imagemap" "string filename" "x.png"', NotImplementedError),
-    ('Material "matte" "texture Kd" "t"', NotImplementedError),
+    ('Texture "t" "color" "checkerboard"', NotImplementedError),
+    ('Material "matte" "texture Kd" "t"\n'
+     'Shape "trianglemesh" "point3 P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2]', ValueError),
+    ('Texture "s" "color" "scale" "rgb tex2" [1 1 1]\nMaterial "matte" "texture Kd" "s"\n'
+     'Shape "trianglemesh" "point3 P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2]',
+     NotImplementedError),
     ('LightSource "distant" "rgb L" [1 1 1]', NotImplementedError),
     ('Shape "trianglemesh" "point3 P" [0 0 0 1 0 0 0 1 0] "integer indices" [0 1 2]', ValueError),
     ('NamedMaterial "nope"', ValueError),

@@ -486,6 +486,33 @@ class PathTracer:
         _lib.call("wl_scene_create", ctx.h, accel.h, meshes, n, larr, len(lights),
                   C.byref(cam), bg, C.byref(h))
         self.h = h
+        if getattr(scene, "textures", None):
+            self.set_textures(scene.textures, [(m.uv, m.texture) for m in scene.meshes])
+
+    def set_textures(self, images, bindings):
+        """images: (H, W, 4) uint8 arrays, top row first; bindings: one
+        (uv (V, 2) or None, image index or None) per mesh.  No images removes
+        the scene's textures."""
+        imgs = [np.ascontiguousarray(im, np.uint8) for im in images]
+        for im in imgs:
+            if im.ndim != 3 or im.shape[2] != 4:
+                raise ValueError("texture images must be (H, W, 4) uint8")
+        iarr = (_lib.TextureImage * max(1, len(imgs)))()
+        for k, im in enumerate(imgs):
+            iarr[k].rgba = im.ctypes.data
+            iarr[k].height, iarr[k].width = im.shape[0], im.shape[1]
+        n = len(bindings)
+        marr = (_lib.MeshTexture * max(1, n))()
+        keep = []
+        for k, (uv, tex) in enumerate(bindings):
+            if uv is not None:
+                u = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+                keep.append(u)
+                marr[k].texcoords = u.ctypes.data
+            else:
+                marr[k].texcoords = None
+            marr[k].texture = WL_INVALID_INDEX if tex is None else int(tex)
+        _lib.call("wl_scene_set_textures", self.h, iarr, len(imgs), marr, n)
 
     def render(self, cfg, film=None, stats=True):
         torch = _torch()

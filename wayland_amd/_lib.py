@@ -49,6 +49,14 @@ class MeshShading(C.Structure):
                 ("area_light", C.c_uint32), ("pad_", C.c_uint32 * 3)]
 
 
+class TextureImage(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class MeshTexture(C.Structure):
+    _fields_ = [("texcoords", C.c_void_p), ("texture", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class AreaLight(C.Structure):
     _fields_ = [("mesh", C.c_uint32), ("L", C.c_float * 3), ("two_sided", C.c_uint32)]
 
@@ -170,6 +178,8 @@ SIGNATURES = {
                                   C.POINTER(C.c_float), C.POINTER(_VP)]),
     "wl_scene_destroy": (C.c_int, [_VP]),
     "wl_scene_set_camera": (C.c_int, [_VP, C.POINTER(PinholeCamera)]),
+    "wl_scene_set_textures": (C.c_int, [_VP, C.POINTER(TextureImage), _U32,
+                                        C.POINTER(MeshTexture), _U32]),
     "wl_pt_render": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, C.POINTER(PTStats), _VP]),
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
     "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),

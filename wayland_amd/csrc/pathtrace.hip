@@ -413,6 +413,18 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 else
                 {
                     const Mat mat = load_mat(S.materials + M.material);
+                    // texture colour (PathTracing.cu:261-268)
+                    V3 tex = mk(1, 1, 1);
+                    if (M.texture != WL_INVALID)
+                    {
+                        const float4 *ur = reinterpret_cast<const float4 *>(S.uvs) + 2 * (size_t)prim;
+                        const float4 u01 = ur[0], u2 = ur[1];
+                        const float w0 = 1 - b1 - b2;
+                        const float tu = (u01.x * w0 + u01.z * b1) + u2.x * b2;
+                        const float tv = (u01.y * w0 + u01.w * b1) + u2.y * b2;
+                        const PtTexture T = S.textures[M.texture];
+                        tex = tex_bilinear(S.texels + T.offset, T.width, T.height, tu, tv);
+                    }
                     if (mat.trans < 0.1f)
                     {
                         if (dot(Ng, rayDir) > 0)
@@ -458,7 +470,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     {
                         throughput = throughput * lastTraceTerm;
                         const float cosWeight = dot(nd, Ns);
-                        V3 bsdf = eval_disney(mat, Ns, Ng, -rayDir, nd, mk(1, 1, 1));
+                        V3 bsdf = eval_disney(mat, Ns, Ng, -rayDir, nd, tex);
                         bsdf = clamp3(bsdf, 0.f, 1e30f);
                         lastTraceTerm = bsdf * fabsf(cosWeight) / pdf;
                         const V3 lastRayDir = rayDir;
@@ -534,7 +546,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                                         uptPdf = P.galpha * guide_pdf(gm, visRay) +
                                                  (1.0f - P.galpha) * uptPdf;
                                 }
-                                V3 bv = eval_disney(mat, Ns, Ng, -lastRayDir, visRay, mk(1, 1, 1));
+                                V3 bv = eval_disney(mat, Ns, Ng, -lastRayDir, visRay, tex);
                                 bv = clamp3(bv, 0.f, 1e30f);
                                 contrib = throughput * bv * fabsf(cosTheta2) * ld3(lt.L) / neePdf *
                                           (1 - uptPdf / (uptPdf + neePdf));

@@ -390,4 +390,42 @@ WL_DEV float aces1(float r)
     return clampf(v, 0.0f, 1.0f);
 }
 
+// Texture fetch with the reference's texture object settings
+// (DeviceManager.cpp:205-220): normalized coordinates, wrap addressing,
+// bilinear filtering, texels read as u8 / 255.  The filter weights keep 8
+// fractional bits (the fixed-point precision the CUDA programming guide states
+// for hardware linear filtering), rounded to nearest.  Texels are RGBA8 words,
+// rows top to bottom; a non-finite coordinate reads as 0.
+WL_DEV float tex_weight_q8(float f) { return floorf(f * 256.0f + 0.5f) * (1.0f / 256.0f); }
+
+WL_DEV V3 texel_rgb(uint32_t c)
+{
+    return mk((float)(c & 255u) / 255.0f, (float)((c >> 8) & 255u) / 255.0f,
+              (float)((c >> 16) & 255u) / 255.0f);
+}
+
+WL_DEV V3 tex_bilinear(const uint32_t *texels, uint32_t w, uint32_t h, float u, float v)
+{
+    float fu = u - floorf(u), fv = v - floorf(v);
+    if (!(fu >= 0.0f && fu <= 1.0f))
+        fu = 0.0f;
+    if (!(fv >= 0.0f && fv <= 1.0f))
+        fv = 0.0f;
+    const float x = fu * (float)w - 0.5f, y = fv * (float)h - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = tex_weight_q8(x - fx), b = tex_weight_q8(y - fy);
+    // x in [-0.5, w - 0.5] so the lower texel is in [-1, w - 1]
+    int i0 = (int)fx, j0 = (int)fy;
+    i0 = i0 < 0 ? i0 + (int)w : i0;
+    j0 = j0 < 0 ? j0 + (int)h : j0;
+    const int i1 = i0 + 1 == (int)w ? 0 : i0 + 1;
+    const int j1 = j0 + 1 == (int)h ? 0 : j0 + 1;
+    const V3 t00 = texel_rgb(texels[(size_t)j0 * w + i0]);
+    const V3 t10 = texel_rgb(texels[(size_t)j0 * w + i1]);
+    const V3 t01 = texel_rgb(texels[(size_t)j1 * w + i0]);
+    const V3 t11 = texel_rgb(texels[(size_t)j1 * w + i1]);
+    return ((t00 * ((1.0f - a) * (1.0f - b)) + t10 * (a * (1.0f - b))) + t01 * ((1.0f - a) * b)) +
+           t11 * (a * b);
+}
+
 } // namespace wl

@@ -70,6 +70,11 @@ struct wl_scene_s
     float box_lo[3] = { 0, 0, 0 }, box_inv[3] = { 1, 1, 1 }; // scene bounds
     uint32_t gcapacity = 0;                   // paths covered by d_gbatch
     void *d_gbatch = nullptr;                 // gout, gacc, grec
+    // textures (wl_scene_set_textures)
+    std::vector<uint32_t> mesh_texture;       // image index per mesh or WL_INVALID_INDEX
+    float *d_uvs = nullptr;                   // 8 floats per primitive (3 x uv + pad)
+    void *d_textures = nullptr;               // wl::PtTexture[num_images]
+    uint32_t *d_texels = nullptr;             // RGBA8 texels, images back to back
 };
 
 struct wl_guide_s

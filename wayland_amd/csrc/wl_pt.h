@@ -16,6 +16,15 @@ struct PtMesh
     uint32_t light;     // area light id or WL_INVALID_INDEX
     uint32_t material;  // index into materials
     uint32_t tri_offset;
+    uint32_t texture;   // index into textures or WL_INVALID_INDEX
+};
+
+// One RGBA8 image in PtScene::texels (rows top to bottom)
+struct PtTexture
+{
+    uint32_t offset;    // first texel
+    uint32_t width, height;
+    uint32_t pad;
 };
 
 struct PtLight
@@ -42,6 +51,9 @@ struct PtScene
     const PtMesh *meshes;
     const wl_disney_material *materials;
     const PtLight *lights;
+    const float *uvs;           // 8 floats per primitive: uv of v0, v1, v2, pad
+    const PtTexture *textures;
+    const uint32_t *texels;
     uint32_t bvh_depth;
 };
 

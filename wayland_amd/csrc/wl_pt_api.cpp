@@ -69,6 +69,9 @@ PtScene pt_scene_view(wl_scene s)
     S.meshes = (const PtMesh *)s->d_meshes;
     S.materials = s->d_materials;
     S.lights = (const PtLight *)s->d_lights;
+    S.uvs = s->d_uvs;
+    S.textures = (const PtTexture *)s->d_textures;
+    S.texels = s->d_texels;
     S.bvh_depth = s->accel->info.depth;
     return S;
 }
@@ -253,6 +256,7 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
         hm[m].light = meshes[m].area_light;
         hm[m].material = m;
         hm[m].tri_offset = off;
+        hm[m].texture = WL_INVALID_INDEX;
         for (uint32_t t = 0; t < cnt; ++t)
         {
             const size_t g = (size_t)off + t;
@@ -311,6 +315,7 @@ wl_status wl_scene_create(wl_context ctx, wl_accel accel,
     s->accel = accel;
     s->num_meshes = num_meshes;
     s->num_lights = num_lights;
+    s->mesh_texture.assign(num_meshes, WL_INVALID_INDEX);
     if (camera)
         s->camera = *camera;
     if (bg)
@@ -363,9 +368,121 @@ wl_status wl_scene_destroy(wl_scene s)
     hipFree(s->d_counters);
     hipFree(s->d_stats);
     hipFree(s->d_gbatch);
+    hipFree(s->d_uvs);
+    hipFree(s->d_textures);
+    hipFree(s->d_texels);
     wl_context ctx = s->ctx;
     delete s;
     wl::ctx_release(ctx);
+    return WL_SUCCESS;
+}
+
+wl_status wl_scene_set_textures(wl_scene s, const wl_texture_image *images, uint32_t num_images,
+                                const wl_mesh_texture *meshes, uint32_t num_meshes)
+{
+    if (!s)
+        return set_error(WL_ERROR_INVALID_VALUE, "scene is NULL");
+    if (num_images && !images)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL images");
+    if (num_images && (!meshes || num_meshes != s->num_meshes))
+        return set_error(WL_ERROR_INVALID_VALUE, "one wl_mesh_texture per build input is required");
+    const wl_accel a = s->accel;
+    std::vector<wl::PtTexture> tab(std::max<uint32_t>(1, num_images));
+    uint64_t texels = 0;
+    for (uint32_t i = 0; i < num_images; ++i)
+    {
+        const wl_texture_image &im = images[i];
+        if (!im.rgba || im.width == 0 || im.height == 0)
+            return set_error(WL_ERROR_INVALID_VALUE, "empty texture image");
+        if (im.width > 65536 || im.height > 65536)
+            return set_error(WL_ERROR_LIMIT_EXCEEDED, "texture side above 65536");
+        tab[i] = wl::PtTexture{ (uint32_t)texels, im.width, im.height, 0u };
+        texels += (uint64_t)im.width * im.height;
+        if (texels > 0xFFFFFFFFull)
+            return set_error(WL_ERROR_LIMIT_EXCEEDED, "more than 2^32 texels");
+    }
+    std::vector<uint32_t> tex_of(s->num_meshes, WL_INVALID_INDEX);
+    for (uint32_t m = 0; num_images && m < s->num_meshes; ++m)
+    {
+        if (meshes[m].texture != WL_INVALID_INDEX && meshes[m].texture >= num_images)
+            return set_error(WL_ERROR_INVALID_VALUE, "mesh texture index out of range");
+        tex_of[m] = meshes[m].texture;
+    }
+    // texcoords gathered per primitive, as normals are (zeros for a mesh without)
+    std::vector<float> uvs;
+    std::vector<uint32_t> pix;
+    if (num_images)
+    {
+        uvs.assign(8 * std::max<uint64_t>(1, a->num_tris), 0.0f);
+        for (uint32_t m = 0; m < s->num_meshes; ++m)
+        {
+            if (!meshes[m].texcoords)
+                continue;
+            const uint32_t off = a->input_offsets[m], cnt = a->input_tri_counts[m];
+            for (uint32_t t = 0; t < cnt; ++t)
+            {
+                const size_t g = (size_t)off + t;
+                for (int k = 0; k < 3; ++k)
+                {
+                    const uint32_t v = a->h_indices[3 * g + k];
+                    uvs[8 * g + 2 * k] = meshes[m].texcoords[2 * (size_t)v];
+                    uvs[8 * g + 2 * k + 1] = meshes[m].texcoords[2 * (size_t)v + 1];
+                }
+            }
+        }
+        pix.resize((size_t)texels);
+        for (uint32_t i = 0; i < num_images; ++i)
+            std::memcpy(pix.data() + tab[i].offset, images[i].rgba,
+                        4 * (size_t)images[i].width * images[i].height);
+    }
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    // running renders may still read the old textures
+    WL_HIP_CHECK(hipDeviceSynchronize());
+    std::vector<wl::PtMesh> hm(std::max<uint32_t>(1, s->num_meshes));
+    WL_HIP_CHECK(hipMemcpy(hm.data(), s->d_meshes, hm.size() * sizeof(wl::PtMesh),
+                           hipMemcpyDeviceToHost));
+    hipFree(s->d_uvs);
+    hipFree(s->d_textures);
+    hipFree(s->d_texels);
+    s->d_uvs = nullptr;
+    s->d_textures = nullptr;
+    s->d_texels = nullptr;
+    hipError_t e = hipSuccess;
+    auto up = [&](void **dst, const void *src, size_t bytes) {
+        if (e != hipSuccess)
+            return;
+        e = hipMalloc(dst, bytes);
+        if (e == hipSuccess)
+            e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    if (num_images)
+    {
+        up((void **)&s->d_uvs, uvs.data(), uvs.size() * 4);
+        up(&s->d_textures, tab.data(), tab.size() * sizeof(wl::PtTexture));
+        up((void **)&s->d_texels, pix.data(), pix.size() * 4);
+    }
+    // the texture field of the per-mesh records; after a failed upload the
+    // scene is left without textures
+    if (e != hipSuccess)
+        std::fill(tex_of.begin(), tex_of.end(), WL_INVALID_INDEX);
+    for (uint32_t m = 0; m < s->num_meshes; ++m)
+        hm[m].texture = tex_of[m];
+    const hipError_t e2 = hipMemcpy(s->d_meshes, hm.data(), hm.size() * sizeof(wl::PtMesh),
+                                    hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = e2;
+    if (e != hipSuccess)
+    {
+        hipFree(s->d_uvs);
+        hipFree(s->d_textures);
+        hipFree(s->d_texels);
+        s->d_uvs = nullptr;
+        s->d_textures = nullptr;
+        s->d_texels = nullptr;
+        s->mesh_texture.assign(s->num_meshes, WL_INVALID_INDEX);
+        return wl::hip_error(e, "texture upload");
+    }
+    s->mesh_texture = tex_of;
     return WL_SUCCESS;
 }
 

@@ -24,9 +24,15 @@ Defaults for parameters a file leaves out are the vendored parser's
 750-751, 795-798), which the reference reads: matte Kd 1, disney color 1,
 plastic Kd = Ks = 0.25 and roughness 0.1, glass Kr = Kt = 1 and eta 1.5, area
 light L 1 one-sided, perspective fov 90, film 640 x 480.
+Textures: Texture "imagemap" (decoded to RGBA8, top row first, as stbi_load
+does in Texture.cpp:13-16) bound through "texture Kd" (matte, plastic),
+"texture Kt" (glass) or "texture color" (disney), with the mesh's "uv"/"st"
+texcoords (zero when absent, as the reference's zero-filled buffer gives);
+"scale" textures are skipped as TransformTexture skips them
+(SceneManager.cpp:157-159).
 Beyond the reference: Shape "plymesh" (ascii or binary PLY with x y z, optional
 nx ny nz, and polygon faces fanned into triangles; SceneManager.cpp:207-221 has
-this path commented out) and Include.  Textures, other shapes, materials and
+this path commented out) and Include.  Other textures, shapes, materials and
 lights raise, as the reference does; a shape with no material raises (the
 reference would index past its material list); ReverseOrientation is accepted
 and, as in the reference, has no effect.
@@ -114,27 +120,43 @@ def _rgb(params, name, default):
     raise NotImplementedError(f"'{p.type} {name}' (only rgb) is not supported")
 
 
+def _color(params, name, default):
+    """A texturable colour: (rgb, texture name or None).  A textured
+    parameter keeps the default value, which the texel then multiplies
+    (minipbrt's ColorTex holds both; Material.cpp:33-107 takes the value and
+    the texture id)."""
+    p = params.get(name)
+    if p is not None and p.type == "texture":
+        return np.asarray(default, np.float32), str(p.values[0])
+    return _rgb(params, name, default), None
+
+
 def _float(params, name, default):
     return float(_get(params, name, [default])[0])
 
 
-def _material(mtype: str, params) -> np.ndarray:
-    """pbrt material -> the reference's Disney parameters (Material.cpp)."""
+def _material(mtype: str, params):
+    """pbrt material -> (the reference's Disney parameters, texture name or
+    None) as Material.cpp maps them; the texture is Kd's for matte and
+    plastic, Kt's for glass and color's for disney."""
     if mtype == "matte":
-        return matte_material(_rgb(params, "Kd", (1.0, 1.0, 1.0)))
+        kd, tex = _color(params, "Kd", (1.0, 1.0, 1.0))
+        return matte_material(kd), tex
     if mtype == "plastic":
-        kd = _rgb(params, "Kd", (0.25, 0.25, 0.25))
+        kd, tex = _color(params, "Kd", (0.25, 0.25, 0.25))
         ks = _rgb(params, "Ks", (0.25, 0.25, 0.25))
         ks_len = np.float32(np.sqrt(np.float32(np.dot(ks, ks))))
         return disney_material(color=kd + ks, metallic=float(np.clip(ks_len, 0, 1)),
                                roughness=_float(params, "roughness", 0.1), sheenTint=0.5,
                                specular=0.0, specularTint=0.0, eta=1.5, trans=0.0,
-                               clearcoat=0.0, clearcoatgloss=1.0)
+                               clearcoat=0.0, clearcoatgloss=1.0), tex
     if mtype == "glass":
-        return glass_material(_rgb(params, "Kr", (1, 1, 1)), _rgb(params, "Kt", (1, 1, 1)),
-                              eta=_float(params, "eta", 1.5))
+        kt, tex = _color(params, "Kt", (1, 1, 1))
+        return glass_material(_rgb(params, "Kr", (1, 1, 1)), kt,
+                              eta=_float(params, "eta", 1.5)), tex
     if mtype == "disney":
-        return disney_material(color=_rgb(params, "color", (1.0, 1.0, 1.0)),
+        color, tex = _color(params, "color", (1.0, 1.0, 1.0))
+        return disney_material(color=color,
                                metallic=_float(params, "metallic", 0.0),
                                roughness=_float(params, "roughness", 0.5),
                                specular=0.0,
@@ -146,14 +168,15 @@ def _material(mtype: str, params) -> np.ndarray:
                                sheen=_float(params, "sheen", 0.0),
                                sheenTint=_float(params, "sheentint", 0.5),
                                clearcoat=_float(params, "clearcoat", 0.0),
-                               clearcoatgloss=_float(params, "clearcoatgloss", 1.0))
+                               clearcoatgloss=_float(params, "clearcoatgloss", 1.0)), tex
     raise NotImplementedError(f"material type '{mtype}' is not supported")
 
 
-def read_ply(path: str):
+def read_ply(path: str, with_uv: bool = False):
     """PLY mesh (ascii or binary_little_endian / binary_big_endian):
-    returns positions (N, 3), normals (N, 3) or None, triangle indices (M, 3);
-    quads are split into two triangles."""
+    returns positions (N, 3), normals (N, 3) or None, triangle indices (M, 3)
+    and, with with_uv, texcoords (N, 2) or None (properties u v, s t or
+    texture_u texture_v); polygons are fanned into triangles."""
     with open(path, "rb") as f:
         data = f.read()
     end = data.index(b"end_header")
@@ -181,7 +204,7 @@ def read_ply(path: str):
     pos = 0
     words = body.split() if fmt == "ascii" else None
     wi = 0
-    verts = normals = faces = None
+    verts = normals = faces = uv = None
     for name, count, props in elements:
         if name == "vertex":
             names = [p[-1] for p in props]
@@ -199,6 +222,10 @@ def read_ply(path: str):
             verts = arr[:, [col["x"], col["y"], col["z"]]].astype(np.float32)
             if all(n in col for n in ("nx", "ny", "nz")):
                 normals = arr[:, [col["nx"], col["ny"], col["nz"]]].astype(np.float32)
+            for un, vn in (("u", "v"), ("s", "t"), ("texture_u", "texture_v")):
+                if un in col and vn in col:
+                    uv = arr[:, [col[un], col[vn]]].astype(np.float32)
+                    break
         elif name == "face":
             tris = []
             lp = next(p for p in props if p[0] == "list")
@@ -240,6 +267,8 @@ def read_ply(path: str):
                 pos += count * sum(struct.calcsize(sizes[p[0]]) for p in props)
     if verts is None or faces is None:
         raise ValueError(f"{path}: no vertex or face element")
+    if with_uv:
+        return verts, normals, faces, uv
     return verts, normals, faces
 
 
@@ -311,8 +340,10 @@ def load_pbrt(path: str, width: Optional[int] = None, height: Optional[int] = No
 
     ctm = np.eye(4)
     stack = []
-    named: Dict[str, np.ndarray] = {}
-    material = None
+    named: Dict[str, tuple] = {}
+    textures: Dict[str, Optional[int]] = {}   # name -> image index (None: skipped)
+    images: List[np.ndarray] = []
+    material = None  # (Disney parameters, texture name or None)
     area = None  # (L, two_sided)
     fov = 90.0
     cam_to_world = None
@@ -402,9 +433,11 @@ def load_pbrt(path: str, width: Optional[int] = None, height: Optional[int] = No
                 idx = np.asarray(_get(params, "indices", None), np.float64).astype(np.uint32)
                 N = _get(params, "N", None)
                 N = None if N is None else np.asarray(N, np.float32).reshape(-1, 3)
+                UV = _get(params, "uv", None) or _get(params, "st", None)
+                UV = None if UV is None else np.asarray(UV, np.float32).reshape(-1, 2)
             elif stype == "plymesh":
                 fn = os.path.join(base, str(_get(params, "filename", [""])[0]))
-                P, N, idx = read_ply(fn)
+                P, N, idx, UV = read_ply(fn, with_uv=True)
                 notes.append(f"plymesh {os.path.basename(fn)}: loaded (the reference's "
                              "SceneManager has this path disabled)")
             else:
@@ -413,12 +446,33 @@ def load_pbrt(path: str, width: Optional[int] = None, height: Optional[int] = No
                 raise ValueError(f"shape {len(meshes)} has no material")
             verts = _transform_points(ctm, P, 1.0)
             normals = None if N is None else _transform_points(ctm, N, 1.0)
-            meshes.append(Mesh(verts, idx.reshape(-1, 3).astype(np.uint32), normals, material,
+            tex = None
+            if material[1] is not None:
+                if material[1] not in textures:
+                    raise ValueError(f"unknown texture '{material[1]}'")
+                tex = textures[material[1]]
+                if tex is None:
+                    raise NotImplementedError(f"material uses the skipped texture '{material[1]}'")
+            meshes.append(Mesh(verts, idx.reshape(-1, 3).astype(np.uint32), normals, material[0],
                                light_L=None if area is None else tuple(float(x) for x in area[0]),
-                               two_sided=False if area is None else area[1]))
+                               two_sided=False if area is None else area[1], uv=UV, texture=tex))
+        elif d == "Texture":
+            tname, tclass = toks[i].strip('"'), toks[i + 2].strip('"')
+            params, i = _parse_params(toks, i + 3)
+            if tclass == "imagemap":
+                fn = os.path.join(base, str(_get(params, "filename", [""])[0]))
+                textures[tname] = len(images)
+                images.append(load_image_rgba(fn))
+            elif tclass == "scale":
+                # TransformTexture leaves scale textures out (SceneManager.cpp:157-159)
+                textures[tname] = None
+                notes.append(f"texture '{tname}': scale textures are skipped, as the reference "
+                             "skips them")
+            else:
+                raise NotImplementedError(f"texture class '{tclass}' is not supported")
         elif d == "ReverseOrientation":
             pass
-        elif d in ("Texture", "MakeNamedMedium", "MediumInterface", "LightSource",
+        elif d in ("MakeNamedMedium", "MediumInterface", "LightSource",
                    "ObjectBegin", "ObjectInstance",
                    "CoordinateSystem", "CoordSysTransform", "ActiveTransform",
                    "TransformTimes"):
@@ -431,8 +485,18 @@ def load_pbrt(path: str, width: Optional[int] = None, height: Optional[int] = No
     H = height or film_h
     camera = camera_from_pbrt(cam_to_world, fov, W, H)
     name = os.path.splitext(os.path.basename(path))[0]
-    return PbrtScene(Scene(meshes, camera, (0.0, 0.0, 0.0), name), W, H, fov, cam_to_world,
-                     notes)
+    return PbrtScene(Scene(meshes, camera, (0.0, 0.0, 0.0), name, images), W, H, fov,
+                     cam_to_world, notes)
+
+
+def load_image_rgba(path: str) -> np.ndarray:
+    """An image file as (H, W, 4) uint8, top row first, grey and RGB widened to
+    RGBA: what stbi_load(path, ..., 4) gives Texture::TexLoadImage
+    (Texture.cpp:13-16).  Decoded with Pillow; a JPEG decoder other than stb's
+    may differ from it by a unit in a few texels."""
+    from PIL import Image
+    with Image.open(path) as im:
+        return np.ascontiguousarray(np.asarray(im.convert("RGBA"), np.uint8))
 
 
 def camera_from_pbrt(cam_to_world: np.ndarray, fov: float, width: int, height: int) -> Camera:

@@ -72,6 +72,8 @@ class Mesh:
     material: np.ndarray = field(default_factory=lambda: disney_material())
     light_L: Optional[tuple] = None      # emitted radiance if an area light
     two_sided: bool = False
+    uv: Optional[np.ndarray] = None      # (V, 2) float32 texcoords or None
+    texture: Optional[int] = None        # index into Scene.textures or None
 
 
 @dataclass
@@ -108,6 +110,7 @@ class Scene:
     camera: Optional[Camera] = None
     bg: tuple = (0.0, 0.0, 0.0)
     name: str = "scene"
+    textures: List[np.ndarray] = field(default_factory=list)  # (H, W, 4) uint8, top row first
 
     @property
     def num_triangles(self) -> int:
@@ -155,6 +158,28 @@ class Scene:
             light_L=np.stack(lights_L) if lights_L else np.zeros((0, 3), np.float32),
             light_two_sided=np.array(lights_2s, u32),
             bg=np.asarray(self.bg, np.float32),
+        )
+        # textures: texcoords indexed as vertices (zeros for a mesh without),
+        # images back to back as RGBA8 words
+        tex_off, tex_size, texels = [], [], []
+        n = 0
+        for img in self.textures:
+            img = np.ascontiguousarray(img, np.uint8)
+            h, w = img.shape[:2]
+            tex_off.append(n)
+            tex_size.append((w, h))
+            texels.append(img.reshape(-1, 4))
+            n += w * h
+        uvs = [np.ascontiguousarray(m.uv, np.float32).reshape(-1, 2) if m.uv is not None
+               else np.zeros((np.asarray(m.vertices).reshape(-1, 3).shape[0], 2), np.float32)
+               for m in self.meshes]
+        out.update(
+            texcoords=np.concatenate(uvs) if uvs else np.zeros((0, 2), np.float32),
+            mesh_texture=np.array([INVALID if (m.texture is None or not self.textures)
+                                   else m.texture for m in self.meshes], u32),
+            texture_offset=np.array(tex_off, u32),
+            texture_size=np.array(tex_size, u32).reshape(-1, 2),
+            texels=np.concatenate(texels) if texels else np.zeros((0, 4), np.uint8),
         )
         return out
 
