@@ -89,8 +89,8 @@ def test_pbrt_imagemap_per_sample_parity(gpu_ctx, tmp_path):
     img[..., 3] = 255
     Image.fromarray(img, "RGBA").save(tmp_path / "wall.png")
     src = open(os.path.join(os.path.dirname(__file__), "golden", "cbox.pbrt")).read()
-    src = src.replace('NamedMaterial "BackWall"\n',
-                      'Texture "w" "color" "imagemap" "string filename" [ "wall.png" ]\n'
+    src = src.replace('\nNamedMaterial "BackWall"\n',
+                      '\nTexture "w" "color" "imagemap" "string filename" [ "wall.png" ]\n'
                       'MakeNamedMaterial "TexWall" "string type" [ "matte" ] "texture Kd" [ "w" ]\n'
                       'NamedMaterial "TexWall"\n', 1)
     (tmp_path / "t.pbrt").write_text(src)
