@@ -467,6 +467,16 @@ wl_status wl_sbt_create(wl_pipeline p, const void *raygen_record, size_t raygen_
                         uint32_t hit_count, size_t hit_stride, wl_sbt *out);
 wl_status wl_sbt_destroy(wl_sbt sbt);
 
+/* A texture object for PathTracing hit records: cudaCreateTextureObject over
+ * a uchar4 array with wrap addressing and linear filtering
+ * (DeviceManager.cpp:192-227).  The handle goes into
+ * wl_pathtracing_hit_data::texture with hasTexture = 1, and texcoords points
+ * at 2 floats per vertex on the device (NULL reads (0, 0)).  The library
+ * keeps its own copy of the image; a scene derived from an SBT copies it
+ * again, so the object may be destroyed after the first launch. */
+wl_status wl_texture_create(wl_context ctx, const wl_texture_image *image, uint64_t *texture);
+wl_status wl_texture_destroy(uint64_t texture);
+
 /* The value a LaunchParams `traversable` field holds for this accel. */
 wl_status wl_accel_traversable(wl_accel accel, uint64_t *handle);
 

@@ -223,3 +223,76 @@ def test_pathtracing_program_launch_matches_oracle(gpu_ctx):
     assert close.all(-1).mean() >= 0.999
     c = col.cpu().numpy()
     assert (np.abs(c.astype(int) - ocol.astype(int)) <= 1).all(-1).mean() >= 0.999
+
+
+@pytest.mark.gpu
+def test_pathtracing_program_with_texture_objects(gpu_ctx):
+    """Hit records with hasTexture = 1, a wl_texture_create handle and device
+    texcoords (PathTracingProgramManager.cpp:97-119) render the oracle's
+    textured frames."""
+    import torch
+    import wayland_amd as wl
+    from test_gpu_texture import textured_cornell
+    from wayland_amd import _lib
+    print("case: PathTracing program with texture objects, cornell 32x32, 2 frames", flush=True)
+    W = H = 32
+    sc = textured_cornell(W, H)
+    accel = wl.Accel(gpu_ctx, sc)
+    pipe = wl.Pipeline(gpu_ctx, PT_SOURCE)
+    texs = [wl.Texture(gpu_ctx, img) for img in sc.textures]
+    keep, hit, lights = [], [], []
+    for i, m in enumerate(sc.meshes):
+        ind = torch.from_numpy(m.indices.astype(np.int32)).cuda()
+        ver = torch.from_numpy(m.vertices).cuda()
+        nrm = torch.from_numpy(m.normals).cuda()
+        keep += [ind, ver, nrm]
+        d = _lib.PathTracingHitData()
+        d.hasNormal = 1
+        d.meshID = i
+        d.areaLightID = len(lights) if m.light_L is not None else 0xFFFFFFFF
+        d.disneyMat[:] = [float(x) for x in m.material[:15]]
+        d.indices, d.vertices, d.normals = ind.data_ptr(), ver.data_ptr(), nrm.data_ptr()
+        if m.texture is not None:
+            d.hasTexture = 1
+            d.texture = texs[m.texture].handle
+            if m.uv is not None:
+                uv = torch.from_numpy(np.ascontiguousarray(m.uv, np.float32)).cuda()
+                keep.append(uv)
+                d.texcoords = uv.data_ptr()
+        hit.append(pipe.record(3, d))
+        hit.append(pipe.record(4, _lib.PathTracingHitData()))
+        if m.light_L is not None:
+            L = _lib.DeviceAreaLight()
+            L.twoSided, L.hasNormal, L.triangleNum = 0, 1, m.indices.shape[0]
+            L.L[:] = list(m.light_L)
+            L.vertices, L.normals, L.indices = ver.data_ptr(), nrm.data_ptr(), ind.data_ptr()
+            lights.append(L)
+    larr = (_lib.DeviceAreaLight * len(lights))(*lights)
+    dl = torch.from_numpy(np.frombuffer(bytes(larr), np.uint8).copy()).cuda()
+    miss = C.c_float * 3
+    sbt = wl.SBT(pipe, pipe.record(0), [pipe.record(1, miss(0, 0, 0)),
+                                        pipe.record(2, miss(0, 0, 0))], hit)
+    rad = torch.zeros((W * H, 4), dtype=torch.float32, device="cuda")
+    col = torch.zeros((W * H, 4), dtype=torch.uint8, device="cuda")
+    p = _lib.PathTracingParams()
+    p.fbSize[:] = [W, H]
+    p.radianceBuffer, p.colorBuffer = rad.data_ptr(), col.data_ptr()
+    cam = sc.camera
+    p.camera.pos[:], p.camera.lookat[:] = list(cam.pos), list(cam.lookat)
+    p.camera.up[:], p.camera.right[:] = list(cam.up), list(cam.right)
+    p.areaLightCount, p.areaLights = len(lights), dl.data_ptr()
+    p.traversable = wl.traversable(accel)
+    opt = ob.OraclePT(sc)
+    oacc = np.zeros((W * H, 4), np.float32)
+    ocol = np.zeros((W * H, 4), np.uint8)
+    for f in range(2):
+        p.frameID = f
+        wl.launch(pipe, p, sbt, W, H)
+        o, _ = opt.render_frame(opt.config(W, H), f)
+        ob.film_update(oacc, ocol, o, f)
+    torch.cuda.synchronize()
+    for t in texs:
+        t.close()
+    a = rad.cpu().numpy()
+    close = np.abs(a - oacc) <= 1e-5 * np.maximum(np.abs(oacc), 1e-3)
+    assert close.all(-1).mean() >= 0.999

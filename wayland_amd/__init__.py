@@ -242,6 +242,31 @@ class SBT:
             pass
 
 
+class Texture:
+    """A texture object for PathTracing hit records (cudaCreateTextureObject,
+    DeviceManager.cpp:192-227); `handle` goes into the record's texture field."""
+
+    def __init__(self, ctx: Context, image):
+        img = np.ascontiguousarray(image, np.uint8)
+        if img.ndim != 3 or img.shape[2] != 4:
+            raise ValueError("texture images must be (H, W, 4) uint8")
+        desc = _lib.TextureImage(img.ctypes.data, img.shape[1], img.shape[0])
+        h = C.c_uint64()
+        _lib.call("wl_texture_create", ctx.h, C.byref(desc), C.byref(h))
+        self.handle = h.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().wl_texture_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def traversable(accel: Accel) -> int:
     v = C.c_uint64()
     _lib.call("wl_accel_traversable", accel.h, C.byref(v))

@@ -213,6 +213,8 @@ SIGNATURES = {
     "wl_sbt_create": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _U32, C.c_size_t, _VP, _U32,
                                 C.c_size_t, C.POINTER(_VP)]),
     "wl_sbt_destroy": (C.c_int, [_VP]),
+    "wl_texture_create": (C.c_int, [_VP, C.POINTER(TextureImage), C.POINTER(C.c_uint64)]),
+    "wl_texture_destroy": (C.c_int, [C.c_uint64]),
     "wl_accel_traversable": (C.c_int, [_VP, C.POINTER(C.c_uint64)]),
     "wl_launch": (C.c_int, [_VP, _VP, _VP, C.c_size_t, _VP, _U32, _U32, _U32]),
     "wl_float_to_u8": (C.c_int, [_VP, C.c_size_t, _VP, _VP]),

@@ -52,6 +52,23 @@ std::set<wl_accel> &accels()
     return *s;
 }
 
+// texture objects (wl_texture_create): an image copy per handle
+struct TextureObject
+{
+    std::vector<uint8_t> rgba;
+    uint32_t width = 0, height = 0;
+};
+std::mutex &texture_mu()
+{
+    static std::mutex *m = new std::mutex;
+    return *m;
+}
+std::set<TextureObject *> &textures()
+{
+    static std::set<TextureObject *> *s = new std::set<TextureObject *>;
+    return *s;
+}
+
 constexpr uint32_t kHeaderMagic = 0x574C4847u; // "WLHG"
 
 bool is_name_char(char c)
@@ -324,6 +341,40 @@ wl_status wl_sbt_destroy(wl_sbt s)
     return WL_SUCCESS;
 }
 
+wl_status wl_texture_create(wl_context ctx, const wl_texture_image *image, uint64_t *texture)
+{
+    if (!ctx || !image || !texture || !image->rgba)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (image->width == 0 || image->height == 0)
+        return set_error(WL_ERROR_INVALID_VALUE, "empty texture image");
+    if (image->width > 65536 || image->height > 65536)
+        return set_error(WL_ERROR_LIMIT_EXCEEDED, "texture side above 65536");
+    TextureObject *t = new (std::nothrow) TextureObject;
+    if (!t)
+        return set_error(WL_ERROR_OUT_OF_MEMORY, "texture object");
+    t->width = image->width;
+    t->height = image->height;
+    t->rgba.assign(image->rgba, image->rgba + 4 * (size_t)image->width * image->height);
+    {
+        std::lock_guard<std::mutex> g(texture_mu());
+        textures().insert(t);
+    }
+    *texture = reinterpret_cast<uint64_t>(t);
+    return WL_SUCCESS;
+}
+
+wl_status wl_texture_destroy(uint64_t texture)
+{
+    TextureObject *t = reinterpret_cast<TextureObject *>(texture);
+    {
+        std::lock_guard<std::mutex> g(texture_mu());
+        if (!textures().erase(t))
+            return set_error(WL_ERROR_INVALID_VALUE, "unknown texture object");
+    }
+    delete t;
+    return WL_SUCCESS;
+}
+
 wl_status wl_accel_traversable(wl_accel a, uint64_t *handle)
 {
     if (!a || !handle)
@@ -389,14 +440,45 @@ static wl_status derive_scene(wl_pipeline p, wl_sbt sbt, wl_accel a,
         return set_error(WL_ERROR_INVALID_VALUE,
                          "two hit records (radiance, shadow) per build input are required");
     std::vector<wl_mesh_shading> sh(M);
-    std::vector<std::vector<float>> normals(M);
+    std::vector<std::vector<float>> normals(M), texcoords(M);
+    // textures: the objects the records name, each once (held under the
+    // registry lock while copied)
+    std::vector<std::vector<uint8_t>> images;
+    std::vector<wl_texture_image> image_desc;
+    std::vector<wl_mesh_texture> mt(M);
+    std::vector<uint64_t> handles;
     for (size_t m = 0; m < M; ++m)
     {
         wl_pathtracing_hit_data d;
         std::memcpy(&d, sbt->hit.data() + (m * rtc) * sbt->hit_stride + WL_SBT_RECORD_HEADER_SIZE,
                     sizeof d);
+        mt[m].texcoords = nullptr;
+        mt[m].texture = WL_INVALID_INDEX;
+        mt[m].pad_ = 0;
         if (d.hasTexture)
-            return set_error(WL_ERROR_NOT_SUPPORTED, "textured materials are not supported");
+        {
+            size_t k = 0;
+            while (k < handles.size() && handles[k] != d.texture)
+                ++k;
+            if (k == handles.size())
+            {
+                std::lock_guard<std::mutex> g(texture_mu());
+                TextureObject *t = reinterpret_cast<TextureObject *>(d.texture);
+                if (!textures().count(t))
+                    return set_error(WL_ERROR_INVALID_VALUE, "hit record names an unknown texture object");
+                handles.push_back(d.texture);
+                images.push_back(t->rgba);
+                image_desc.push_back(wl_texture_image{ nullptr, t->width, t->height });
+            }
+            mt[m].texture = (uint32_t)k;
+            if (d.texcoords && a->input_num_vertices[m])
+            {
+                texcoords[m].resize(2 * (size_t)a->input_num_vertices[m]);
+                WL_HIP_CHECK(hipMemcpy(texcoords[m].data(), d.texcoords, texcoords[m].size() * 4,
+                                       hipMemcpyDeviceToHost));
+                mt[m].texcoords = texcoords[m].data();
+            }
+        }
         std::memset(&sh[m], 0, sizeof sh[m]);
         std::memcpy(&sh[m].material, d.disneyMat, sizeof d.disneyMat);
         sh[m].area_light = d.areaLightID;
@@ -439,6 +521,18 @@ static wl_status derive_scene(wl_pipeline p, wl_sbt sbt, wl_accel a,
                                    (uint32_t)lights.size(), &prm.camera, bg, &s);
     if (st != WL_SUCCESS)
         return st;
+    if (!images.empty())
+    {
+        for (size_t k = 0; k < images.size(); ++k)
+            image_desc[k].rgba = images[k].data();
+        st = wl_scene_set_textures(s, image_desc.data(), (uint32_t)image_desc.size(), mt.data(),
+                                   (uint32_t)M);
+        if (st != WL_SUCCESS)
+        {
+            wl_scene_destroy(s);
+            return st;
+        }
+    }
     sbt->scene = s;
     sbt->scene_accel = a;
     sbt->scene_lights = prm.areaLights;
