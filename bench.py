@@ -52,6 +52,8 @@ def parse():
     p.add_argument("--guide-runs", type=int, default=3)
     p.add_argument("--serve-requests", type=int, default=64,
                    help="batched-serving requests (0 = skip)")
+    p.add_argument("--scene-file-frames", type=int, default=64,
+                   help="headless frames of cbox.pbrt at 1024^2 (0 = skip the scene-file leg)")
     p.add_argument("--cornell-ref-spp", type=int, default=256,
                    help="FP64 oracle samples per pixel for the Cornell relMSE "
                         "(0 = skip the Cornell leg)")
@@ -303,6 +305,54 @@ def bench_cornell(args, ctx, wl, torch):
     return out
 
 
+def bench_scene_files(args, ctx, wl, torch):
+    """Reference-format inputs: (1) the headless frame loop on the Cornell box
+    read from its pbrt file (tests/golden/cbox.pbrt, 1024 x 1024), one sample
+    per frame through the progressive film with the frame buffer downloaded
+    every frame, as Renderer::Run does; (2) the textured Cornell box against
+    the same box untextured, 1024 x 1024, 16 spp, depth 25."""
+    from wayland_amd import render, scenes
+    res, frames = 1024, args.scene_file_frames
+    setting = render.RendererSetting.from_args(
+        ["-s", os.path.join(ROOT, "tests", "golden", "cbox.pbrt"), "-fr", "4",
+         "-r", str(res), str(res)])
+    r = render.Renderer(setting, ctx=ctx)
+    r.run()  # warm
+    setting.frame_limit = frames
+    n, seconds = r.run()
+    r.close()
+    headless = {"workload": f"cbox.pbrt {res}x{res}, 1 spp per frame, progressive film + "
+                            "u8 frame buffer downloaded per frame", "frames": n,
+                "seconds": seconds, "frames_per_s": n / seconds,
+                "mpaths_per_s": n * res * res / seconds / 1e6}
+    out = {"headless_pbrt": headless}
+    spp = 16
+    rates = {}
+    for name, sc in (("plain", scenes.cornell_box(res, res)),
+                     ("textured", scenes.textured_cornell_box(res, res))):
+        accel = wl.Accel(ctx, sc)
+        pt = wl.PathTracer(ctx, accel, sc)
+        cfg = wl.pt_config(res, res, spp=spp)
+        pt.render(cfg, stats=False)
+        times = []
+        for _ in range(3):
+            film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            pt.render(cfg, film, stats=False)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        rates[name] = res * res * spp / float(np.median(times)) / 1e6
+        pt.close()
+        accel.close()
+    out["textures"] = {"workload": f"cornell box {res}x{res}, {spp} spp, depth 25; back wall "
+                                   "37x23 and floor 16x16 RGBA8 textures, wrap + bilinear",
+                       "mpaths_per_s_textured": rates["textured"],
+                       "mpaths_per_s_untextured": rates["plain"]}
+    log(f"scene files: {out}")
+    return out
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -403,6 +453,10 @@ def main():
     if args.serve_requests > 0:
         serve_out = bench_serve(args, ctx, wl, torch)
 
+    files_out = None
+    if args.scene_file_frames > 0:
+        files_out = bench_scene_files(args, ctx, wl, torch)
+
     cornell_out = None
     if args.cornell_ref_spp > 0:
         cornell_out = bench_cornell(args, ctx, wl, torch)
@@ -447,6 +501,7 @@ def main():
         "guiding": guide_out,
         "cornell_cpu_workload": cornell_out,
         "serving": serve_out,
+        "scene_files": files_out,
     }
     print(json.dumps(out), flush=True)
 

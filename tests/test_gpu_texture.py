@@ -25,37 +25,10 @@ def _frac_within(a, b, rel=REL, floor=1e-6):
     return float((err.reshape(-1, 3).max(1) <= rel).mean())
 
 
-def textured_cornell(w, h):
-    sc = scenes.cornell_box(w, h)
-    rng = np.random.default_rng(9)
-    back = rng.integers(0, 256, (23, 37, 4), dtype=np.uint8)
-    yy, xx = np.mgrid[0:16, 0:16]
-    floor = np.zeros((16, 16, 4), np.uint8)
-    floor[..., 0] = np.where((xx // 4 + yy // 4) % 2 == 0, 240, 30)
-    floor[..., 1] = 8 * xx
-    floor[..., 2] = 8 * yy
-    floor[..., 3] = 255
-    sc.textures = [back, floor]
-    for m in sc.meshes:
-        if m.light_L is not None:
-            continue
-        n = np.asarray(m.vertices).reshape(-1, 3).shape[0]
-        v = np.asarray(m.vertices)
-        if n == 4 and np.allclose(v[:, 2], -1):          # back wall
-            m.uv = np.stack([(v[:, 0] + 1) / 2, 1 - v[:, 1] / 2], 1).astype(np.float32)
-            m.texture = 0
-        elif n == 4 and np.allclose(v[:, 1], 0):         # floor
-            m.uv = np.stack([v[:, 0] * 2 + 0.5, v[:, 2] * 2 + 0.5], 1).astype(np.float32)
-            m.texture = 1
-    boxes = [m for m in sc.meshes if np.asarray(m.vertices).reshape(-1, 3).shape[0] == 24]
-    boxes[0].texture = 0                                 # no uv: reads texel (0, 0) area
-    return sc
-
-
 def test_textured_cornell_per_sample_parity(gpu_ctx):
     import wayland_amd as wl
     print("case: textured cornell 64x64 frames 0-2", flush=True)
-    sc = textured_cornell(64, 64)
+    sc = scenes.textured_cornell_box(64, 64)
     accel = wl.Accel(gpu_ctx, sc)
     pt = wl.PathTracer(gpu_ctx, accel, sc)
     opt = ob.OraclePT(sc)
@@ -113,7 +86,7 @@ def test_pbrt_imagemap_per_sample_parity(gpu_ctx, tmp_path):
 def test_textured_scene_serves_bit_identically(gpu_ctx):
     import wayland_amd as wl
     print("case: serve textured cornell, 2 requests", flush=True)
-    sc = textured_cornell(32, 32)
+    sc = scenes.textured_cornell_box(32, 32)
     accel = wl.Accel(gpu_ctx, sc)
     pt = wl.PathTracer(gpu_ctx, accel, sc)
     reqs = [{"pt": pt, "width": 32, "height": 32, "spp": 3},

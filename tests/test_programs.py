@@ -232,11 +232,10 @@ def test_pathtracing_program_with_texture_objects(gpu_ctx):
     textured frames."""
     import torch
     import wayland_amd as wl
-    from test_gpu_texture import textured_cornell
     from wayland_amd import _lib
     print("case: PathTracing program with texture objects, cornell 32x32, 2 frames", flush=True)
     W = H = 32
-    sc = textured_cornell(W, H)
+    sc = scenes.textured_cornell_box(W, H)
     accel = wl.Accel(gpu_ctx, sc)
     pipe = wl.Pipeline(gpu_ctx, PT_SOURCE)
     texs = [wl.Texture(gpu_ctx, img) for img in sc.textures]

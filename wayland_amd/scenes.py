@@ -14,6 +14,7 @@ oracle and the benchmark all see bit-identical float32 inputs:
   parameters, two quad lights and a constant environment.
 * ``glossy_caustic``     — glass icosphere over a rough-metal floor grid with a
   small quad light behind it.
+* ``textured_cornell_box`` — the Cornell box with two RGBA8 textures.
 
 A scene is a list of meshes plus camera/background; ``Scene.flatten()`` packs it
 into the flat arrays the C ABI and the oracle take (mesh-local indices,
@@ -254,6 +255,37 @@ def cornell_box(width=256, height=256, seed=0, light_L=17.0) -> Scene:
 # --------------------------------------------------------------------------
 # Icospheres
 # --------------------------------------------------------------------------
+def textured_cornell_box(width=256, height=256) -> Scene:
+    """The Cornell box with textures: a 37 x 23 random RGBA8 image on the back
+    wall (uv = the wall's x, y), a 16 x 16 checker-and-gradient image on the
+    floor with texcoords spanning [-1.5, 2.5] (wrap addressing), and the
+    first box bound to the wall image without texcoords (reads (0, 0))."""
+    sc = cornell_box(width, height)
+    rng = np.random.default_rng(9)
+    back = rng.integers(0, 256, (23, 37, 4), dtype=np.uint8)
+    yy, xx = np.mgrid[0:16, 0:16]
+    floor = np.zeros((16, 16, 4), np.uint8)
+    floor[..., 0] = np.where((xx // 4 + yy // 4) % 2 == 0, 240, 30)
+    floor[..., 1] = 8 * xx
+    floor[..., 2] = 8 * yy
+    floor[..., 3] = 255
+    sc.textures = [back, floor]
+    sc.name = "cornell_textured"
+    for m in sc.meshes:
+        if m.light_L is not None:
+            continue
+        v = np.asarray(m.vertices).reshape(-1, 3)
+        if v.shape[0] == 4 and np.allclose(v[:, 2], -1):      # back wall
+            m.uv = np.stack([(v[:, 0] + 1) / 2, 1 - v[:, 1] / 2], 1).astype(np.float32)
+            m.texture = 0
+        elif v.shape[0] == 4 and np.allclose(v[:, 1], 0):     # floor
+            m.uv = np.stack([v[:, 0] * 2 + 0.5, v[:, 2] * 2 + 0.5], 1).astype(np.float32)
+            m.texture = 1
+    boxes = [m for m in sc.meshes if np.asarray(m.vertices).reshape(-1, 3).shape[0] == 24]
+    boxes[0].texture = 0
+    return sc
+
+
 def _icosahedron():
     t = (1.0 + 5.0 ** 0.5) / 2.0
     v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
