@@ -68,9 +68,35 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
     build_hostemu()
+    build_cpp_examples()
     if verbose:
         print("built", LIB, "and", HOSTEMU)
     return LIB
+
+
+CPP_EXAMPLE = os.path.join(ROOT, "tests", "cpp", "example_pathtracer")
+CPP_EXAMPLE_LIB = os.path.join(ROOT, "tests", "cpp", "libexample_pathtracer.so")
+
+
+def build_cpp_examples() -> str:
+    """The examples/PathTracer driver over include/wayland_hip.hpp (tests only):
+    host C++20, linked against the library with an rpath to it."""
+    src = CPP_EXAMPLE + ".cpp"
+    if not os.path.exists(src):
+        return ""
+    common = ["-x", "c++", "-std=c++20", "-O2", src, "-D__HIP_PLATFORM_AMD__",
+              "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+              "-L" + PKG, "-lwayland_hip", "-L/opt/rocm/lib", "-lamdhip64",
+              "-Wl,-rpath,$ORIGIN/../../wayland_amd"]
+    for out, extra in ((CPP_EXAMPLE, ["-DWL_EXAMPLE_EXECUTABLE"]),
+                       (CPP_EXAMPLE_LIB, ["-shared", "-fPIC"])):
+        tmp = out + ".tmp"
+        cmd = [HIPCC, "-o", tmp] + extra + common
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, out)
+    return CPP_EXAMPLE
 
 
 HOSTEMU = os.path.join(PKG, "libwl_hostemu.so")
