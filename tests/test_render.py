@@ -84,3 +84,17 @@ def test_headless_run_needs_a_limit():
 def test_other_programs_raise():
     with pytest.raises(NotImplementedError):
         render.Renderer(render.RendererSetting.from_args(["-s", "x.pbrt", "-p", "AO"]))
+
+
+def test_save_image_cpu_jpeg(tmp_path):
+    from PIL import Image
+    yy, xx = np.mgrid[0:32, 0:48]
+    rgba = np.zeros((32, 48, 4), np.uint8)
+    rgba[..., 0], rgba[..., 1], rgba[..., 2], rgba[..., 3] = 5 * xx, 7 * yy, 128, 255
+    path = str(tmp_path / "img.jpg")
+    render.save_image_cpu(path, 48, 32, rgba.reshape(-1, 4))
+    with Image.open(path) as im:
+        assert im.format == "JPEG" and im.size == (48, 32)
+        dec = np.asarray(im.convert("RGB")).astype(int)
+    # a smooth image survives quality-90 coding to a few units
+    assert np.abs(dec - rgba[..., :3]).mean() < 2.0

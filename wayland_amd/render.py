@@ -132,6 +132,15 @@ def png_bytes(rgba: np.ndarray) -> bytes:
             + chunk(b"IEND", b""))
 
 
+def save_image_cpu(path: str, width: int, height: int, rgba: np.ndarray) -> None:
+    """examples/Common/Common.cpp:23-41 SaveImageCPU: the RGBA8 buffer as a
+    JPEG of quality 90, rows in buffer order (stbi_write_jpg drops alpha).
+    Encoded with Pillow, so the bytes differ from stb's encoder."""
+    from PIL import Image
+    img = np.asarray(rgba, np.uint8).reshape(height, width, 4)
+    Image.fromarray(np.ascontiguousarray(img[..., :3]), "RGB").save(path, "JPEG", quality=90)
+
+
 def flip_frame_buffer(frame_buffer: np.ndarray, width: int, height: int) -> np.ndarray:
     """The whole buffer reversed, then each row reversed back: a vertical
     flip (Renderer.cpp:96-102)."""
