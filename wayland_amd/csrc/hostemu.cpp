@@ -88,8 +88,24 @@ uint64_t wlh_trace(void *p, const float *rays, uint32_t n, float *hits,
                 res = wl::trav_run<true, true, 1>(h->bvh.nodes.data(), h->bvh.tris.data(),
                                                   tr, true, stack, nn, nt);
             else
-                res = wl::trav_run<false, true, 1>(h->bvh.nodes.data(), h->bvh.tris.data(),
-                                                   tr, true, stack, nn, nt);
+            {
+                // the closest-hit kernels' step: one node or one triangle
+                wl::TravState st;
+                wl::trav_init(st, tr, true);
+                for (;;)
+                {
+                    const bool tri = st.trimask != 0;
+                    if (wl::trav_step_one<1>(h->bvh.nodes.data(), h->bvh.tris.data(), tr, st,
+                                             stack))
+                        break;
+                    if (tri)
+                        ++nt;
+                    else
+                        ++nn;
+                }
+                wl::trav_finish(st.res);
+                res = st.res;
+            }
             float *o = hits + 4 * (size_t)i;
             o[0] = res.t;
             std::memcpy(&o[1], &res.prim, 4);

@@ -16,15 +16,20 @@ namespace wl
 
 constexpr int kTraceBlock = 256;
 
+// Rays are read once and hits written once: non-temporal accesses keep them
+// from displacing BVH nodes in the caches.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32_t i, TravRay &r)
 {
-    const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-    const float4 a = rp[0], b = rp[1];
+    const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
+    const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
     trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
 }
 
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_closest_p(
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+void k_trace_closest_p(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits)
 {
@@ -33,8 +38,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest_p(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {
-            reinterpret_cast<float4 *>(hits)[i] =
-                make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
+            const f4v v = { h.t, __uint_as_float(h.prim), h.b1, h.b2 };
+            __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(hits) + i);
         });
 }
 

@@ -12,6 +12,14 @@
 // key space), a triangle mask of the node just visited, and an LDS stack of
 // 4-byte group entries, WL_BVH8_STACK_DEPTH deep, laid out [depth][lane] so
 // the 64 lanes of a wave touch 64 consecutive words.
+//
+// Two step shapes share the node and triangle code.  trav_step visits a node
+// and then tests all of its triangles (any-hit, which stops at the first hit,
+// and the instrumented kernel).  The closest-hit kernels step with
+// trav_node_one / trav_tri_one instead: a lane either visits one node or
+// tests one pending triangle per step, so lanes with several triangles do not
+// hold the whole wave in a triangle loop.  Both visit the same nodes and test
+// the same triangles in the same order, so the results are the same.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -40,10 +48,8 @@ struct TravRay
     float tmin, tmax;
     float inv[3];
     float Sx, Sy, Sz;
-    float okx, oky, okz;   // origin permuted by (kx, ky, kz)
-    int kx, ky, kz;
-    uint32_t xkey;         // slot ^ xkey = visit order key (near first)
-    uint32_t neg;          // bit k: d_k < 0
+    uint32_t kxyz;         // kx | ky << 2 | kz << 4 (the triangle test's axis order)
+    uint32_t neg;          // bit k: d_k < 0; slot ^ neg ^ 7 = visit order key (near first)
 };
 
 WL_HD float sel3(int k, float a, float b, float c)
@@ -75,15 +81,10 @@ WL_HD void trav_setup(TravRay &r, float ox, float oy,
         kx = ky;
         ky = t;
     }
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    r.kxyz = (uint32_t)kx | (uint32_t)ky << 2 | (uint32_t)kz << 4;
     r.Sx = sel3(kx, dx, dy, dz) / dkz;
     r.Sy = sel3(ky, dx, dy, dz) / dkz;
     r.Sz = 1.0f / dkz;
-    r.okx = sel3(kx, ox, oy, oz);
-    r.oky = sel3(ky, ox, oy, oz);
-    r.okz = sel3(kz, ox, oy, oz);
     uint32_t neg = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -91,12 +92,17 @@ WL_HD void trav_setup(TravRay &r, float ox, float oy,
         float di = d[i];
         if (fabsf(di) < 1e-30f)
             di = copysignf(1e-30f, di);
+        // the reciprocals only place box planes, which are padded; the
+        // triangle test uses the correctly rounded Sx, Sy, Sz
+#if defined(__HIP_DEVICE_COMPILE__)
+        r.inv[i] = __builtin_amdgcn_rcpf(di);
+#else
         r.inv[i] = 1.0f / di;
+#endif
         if (r.inv[i] < 0.0f)
             neg |= 1u << i;
     }
     r.neg = neg;
-    r.xkey = neg ^ 7u;
 }
 
 // Watertight test; returns true and (t, b1, b2) for tmin <= t <= tmax.
@@ -106,15 +112,19 @@ WL_HD bool tri_test(const TravRay &r, const float4 w0,
                                          float &b2_out)
 {
     // A = v0 - o, B = v1 - o, C = v2 - o, components permuted (kx, ky, kz)
-    const float Akx = sel3(r.kx, w0.x, w0.y, w0.z) - r.okx;
-    const float Aky = sel3(r.ky, w0.x, w0.y, w0.z) - r.oky;
-    const float Akz = sel3(r.kz, w0.x, w0.y, w0.z) - r.okz;
-    const float Bkx = sel3(r.kx, w0.w, w1.x, w1.y) - r.okx;
-    const float Bky = sel3(r.ky, w0.w, w1.x, w1.y) - r.oky;
-    const float Bkz = sel3(r.kz, w0.w, w1.x, w1.y) - r.okz;
-    const float Ckx = sel3(r.kx, w1.z, w1.w, w2.x) - r.okx;
-    const float Cky = sel3(r.ky, w1.z, w1.w, w2.x) - r.oky;
-    const float Ckz = sel3(r.kz, w1.z, w1.w, w2.x) - r.okz;
+    const int kx = (int)(r.kxyz & 3u), ky = (int)((r.kxyz >> 2) & 3u), kz = (int)(r.kxyz >> 4);
+    const float okx = sel3(kx, r.o[0], r.o[1], r.o[2]);
+    const float oky = sel3(ky, r.o[0], r.o[1], r.o[2]);
+    const float okz = sel3(kz, r.o[0], r.o[1], r.o[2]);
+    const float Akx = sel3(kx, w0.x, w0.y, w0.z) - okx;
+    const float Aky = sel3(ky, w0.x, w0.y, w0.z) - oky;
+    const float Akz = sel3(kz, w0.x, w0.y, w0.z) - okz;
+    const float Bkx = sel3(kx, w0.w, w1.x, w1.y) - okx;
+    const float Bky = sel3(ky, w0.w, w1.x, w1.y) - oky;
+    const float Bkz = sel3(kz, w0.w, w1.x, w1.y) - okz;
+    const float Ckx = sel3(kx, w1.z, w1.w, w2.x) - okx;
+    const float Cky = sel3(ky, w1.z, w1.w, w2.x) - oky;
+    const float Ckz = sel3(kz, w1.z, w1.w, w2.x) - okz;
     const float Ax = Akx - r.Sx * Akz;
     const float Ay = Aky - r.Sy * Akz;
     const float Bx = Bkx - r.Sx * Bkz;
@@ -172,6 +182,7 @@ struct TravState
     uint32_t grp_base, grp_mask, node;
     int sp;
     bool visit;
+    uint32_t trimask, tri_base; // pending triangles (trav_node_one / trav_tri_one)
 };
 
 WL_HD void trav_init(TravState &st, const TravRay &r, bool active)
@@ -187,6 +198,162 @@ WL_HD void trav_init(TravState &st, const TravRay &r, bool active)
     st.grp_mask = 0;
     st.node = 0;
     st.visit = active;
+    st.trimask = 0;
+    st.tri_base = 0;
+}
+
+// The eight child boxes of `node` against [tmin, min(t_best, ...)]: inner
+// hits in key space (bit s ^ neg ^ 7), and the triangle bits of the hit leaves.
+struct NodeHits
+{
+    uint32_t child_base, tri_base, inner, trimask;
+};
+
+WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
+                          const TravRay &r, float t_best)
+{
+    const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
+    const uint4 h0 = np[0];
+    const uint4 h1 = np[1];
+    const uint4 qlx = np[2], qly = np[3], qlz = np[4];
+    const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
+    const float px = wl_u2f(h0.x), py = wl_u2f(h0.y), pz = wl_u2f(h0.z);
+    const uint32_t ew = h0.w;
+    const float sx = wl_u2f((ew & 0xFFu) << 23);
+    const float sy = wl_u2f(((ew >> 8) & 0xFFu) << 23);
+    const float sz = wl_u2f(((ew >> 16) & 0xFFu) << 23);
+    const uint32_t imask = ew >> 24;
+    const uint32_t meta_lo = h1.z, meta_hi = h1.w;
+    // per-axis plane parameters: t = q * sinv + adj
+    const float ax = (px - r.o[0]) * r.inv[0];
+    const float ay = (py - r.o[1]) * r.inv[1];
+    const float az = (pz - r.o[2]) * r.inv[2];
+    const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
+    // near/far quantised planes chosen by the ray's direction signs
+    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
+    const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
+    const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
+    const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
+    const float cull = t_best * 1.00001f;
+    const float tmin = r.tmin;
+    const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
+    const uint32_t fxw[4] = { fqx.x, fqx.y, fqx.z, fqx.w };
+    const uint32_t nyw[4] = { nqy.x, nqy.y, nqy.z, nqy.w };
+    const uint32_t fyw[4] = { fqy.x, fqy.y, fqy.z, fqy.w };
+    const uint32_t nzw[4] = { nqz.x, nqz.y, nqz.z, nqz.w };
+    const uint32_t fzw[4] = { fqz.x, fqz.y, fqz.z, fqz.w };
+    // all eight tests first, without branches: the hit mask
+    uint32_t hitmask = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+    {
+        const int wi = s >> 1;
+        float qnx, qfx, qny, qfy, qnz, qfz;
+        if (s & 1)
+        {
+            qnx = q_hi16(nxw[wi]); qfx = q_hi16(fxw[wi]);
+            qny = q_hi16(nyw[wi]); qfy = q_hi16(fyw[wi]);
+            qnz = q_hi16(nzw[wi]); qfz = q_hi16(fzw[wi]);
+        }
+        else
+        {
+            qnx = q_lo16(nxw[wi]); qfx = q_lo16(fxw[wi]);
+            qny = q_lo16(nyw[wi]); qfy = q_lo16(fyw[wi]);
+            qnz = q_lo16(nzw[wi]); qfz = q_lo16(fzw[wi]);
+        }
+        const float tnx = __builtin_fmaf(qnx, six, ax);
+        const float tfx = __builtin_fmaf(qfx, six, ax);
+        const float tny = __builtin_fmaf(qny, siy, ay);
+        const float tfy = __builtin_fmaf(qfy, siy, ay);
+        const float tnz = __builtin_fmaf(qnz, siz, az);
+        const float tfz = __builtin_fmaf(qfz, siz, az);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+        float tf = fminf(fminf(tfx, tfy), tfz);
+        tf = __builtin_fmaf(fabsf(tf), 4e-6f, tf);
+        hitmask |= (tn <= fminf(tf, cull)) ? (1u << s) : 0u;
+    }
+    NodeHits h;
+    h.child_base = h1.x;
+    h.tri_base = h1.y;
+    // inner hits to key space: bit s -> bit s ^ k, k = neg ^ 7 (xor of a 3-bit index
+    // = swaps of single bits, pairs and nibbles)
+    uint32_t m = hitmask & imask;
+    const uint32_t k = r.neg ^ 7u;
+    m = (k & 1u) ? (((m & 0x55u) << 1) | ((m >> 1) & 0x55u)) : m;
+    m = (k & 2u) ? (((m & 0x33u) << 2) | ((m >> 2) & 0x33u)) : m;
+    m = (k & 4u) ? (((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu)) : m;
+    h.inner = m;
+    // hit leaves: their triangle ranges (few per node, so a loop)
+    uint32_t trimask = 0;
+    uint32_t leaf = hitmask & ~imask;
+    while (leaf)
+    {
+        const uint32_t s = __builtin_ctz(leaf);
+        leaf &= leaf - 1u;
+        const uint32_t mm = ((s < 4 ? meta_lo : meta_hi) >> (8 * (s & 3))) & 0xFFu;
+        // 0xFF marks an empty slot (its inverted box can still pass the
+        // padded test on a tiny node)
+        if (mm != 0xFFu)
+        {
+            const uint32_t off = mm >> 2, cnt = (mm & 3u) + 1u;
+            trimask |= ((1u << cnt) - 1u) << off;
+        }
+    }
+    h.trimask = trimask;
+    return h;
+}
+
+// Test triangle tri_base + i; keeps the closest hit (ties to the smaller
+// primitive id).  Returns true for an accepted hit.
+template <bool ANY, bool AUX>
+WL_HD bool tri_visit(const WlTriRecord *__restrict__ tris, uint32_t tri_base, uint32_t i,
+                     const TravRay &r, TravState &st)
+{
+    const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
+    const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
+    float t, b1, b2;
+    if (tri_test(r, w0, w1, w2, t, b1, b2))
+    {
+        const uint32_t prim = wl_f2u(w2.y);
+        if (ANY || t < st.res.t || (t == st.res.t && prim < st.res.prim))
+        {
+            st.res.t = t;
+            st.res.prim = prim;
+            st.res.b1 = b1;
+            st.res.b2 = b2;
+            if (AUX)
+            {
+                st.res.input = wl_f2u(w2.z);
+                st.res.local = wl_f2u(w2.w);
+            }
+            return true;
+        }
+    }
+    return false;
+}
+
+// Next node from the current group, else from the stack; false when none.
+template <int BLOCK>
+WL_HD bool next_node(const TravRay &r, TravState &st, uint32_t *lds_stack_lane)
+{
+    if (st.grp_mask == 0)
+    {
+        if (st.sp == 0)
+            return false;
+        --st.sp;
+        const uint32_t e = lds_stack_lane[st.sp * BLOCK];
+        st.grp_base = (e >> 8) << 3;
+        st.grp_mask = e & 0xFFu;
+    }
+    const uint32_t pos = __builtin_ctz(st.grp_mask);
+    st.grp_mask &= st.grp_mask - 1u;
+    st.node = st.grp_base + (pos ^ r.neg ^ 7u);
+    if (st.grp_mask)
+    {
+        lds_stack_lane[st.sp * BLOCK] = ((st.grp_base >> 3) << 8) | st.grp_mask;
+        ++st.sp;
+    }
+    return true;
 }
 
 // One step: visit the pending node (8 child boxes), test the triangles of its
@@ -199,93 +366,16 @@ WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
                      uint32_t &n_tris)
 {
     uint32_t trimask = 0, tri_base = 0;
-    const float tmin = r.tmin;
     if (st.visit)
     {
-        const uint32_t node = st.node;
-        WL_TRAV_CHECK(node < wl_trav_nodes_bound, "node index");
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
-        const uint4 h0 = np[0];
-        const uint4 h1 = np[1];
-        const uint4 qlx = np[2], qly = np[3], qlz = np[4];
-        const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
+        WL_TRAV_CHECK(st.node < wl_trav_nodes_bound, "node index");
+        const NodeHits h = node_visit(nodes, st.node, r, st.res.t);
         if (COUNT)
             ++n_nodes;
-        const float px = wl_u2f(h0.x), py = wl_u2f(h0.y), pz = wl_u2f(h0.z);
-        const uint32_t ew = h0.w;
-        const float sx = wl_u2f((ew & 0xFFu) << 23);
-        const float sy = wl_u2f(((ew >> 8) & 0xFFu) << 23);
-        const float sz = wl_u2f(((ew >> 16) & 0xFFu) << 23);
-        const uint32_t imask = ew >> 24;
-        const uint32_t child_base = h1.x;
-        tri_base = h1.y;
-        const uint32_t meta_lo = h1.z, meta_hi = h1.w;
-        // per-axis plane parameters: t = q * sinv + adj
-        const float ax = (px - r.o[0]) * r.inv[0];
-        const float ay = (py - r.o[1]) * r.inv[1];
-        const float az = (pz - r.o[2]) * r.inv[2];
-        const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
-        // near/far quantised planes chosen by the ray's direction signs
-        const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
-        const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
-        const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
-        const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
-        const float cull = st.res.t * 1.00001f;
-        const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
-        const uint32_t fxw[4] = { fqx.x, fqx.y, fqx.z, fqx.w };
-        const uint32_t nyw[4] = { nqy.x, nqy.y, nqy.z, nqy.w };
-        const uint32_t fyw[4] = { fqy.x, fqy.y, fqy.z, fqy.w };
-        const uint32_t nzw[4] = { nqz.x, nqz.y, nqz.z, nqz.w };
-        const uint32_t fzw[4] = { fqz.x, fqz.y, fqz.z, fqz.w };
-        uint32_t inner_hits = 0;
-#pragma unroll
-        for (int s = 0; s < 8; ++s)
-        {
-            const int wi = s >> 1;
-            float qnx, qfx, qny, qfy, qnz, qfz;
-            if (s & 1)
-            {
-                qnx = q_hi16(nxw[wi]); qfx = q_hi16(fxw[wi]);
-                qny = q_hi16(nyw[wi]); qfy = q_hi16(fyw[wi]);
-                qnz = q_hi16(nzw[wi]); qfz = q_hi16(fzw[wi]);
-            }
-            else
-            {
-                qnx = q_lo16(nxw[wi]); qfx = q_lo16(fxw[wi]);
-                qny = q_lo16(nyw[wi]); qfy = q_lo16(fyw[wi]);
-                qnz = q_lo16(nzw[wi]); qfz = q_lo16(fzw[wi]);
-            }
-            const float tnx = __builtin_fmaf(qnx, six, ax);
-            const float tfx = __builtin_fmaf(qfx, six, ax);
-            const float tny = __builtin_fmaf(qny, siy, ay);
-            const float tfy = __builtin_fmaf(qfy, siy, ay);
-            const float tnz = __builtin_fmaf(qnz, siz, az);
-            const float tfz = __builtin_fmaf(qfz, siz, az);
-            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-            float tf = fminf(fminf(tfx, tfy), tfz);
-            tf = __builtin_fmaf(fabsf(tf), 4e-6f, tf);
-            const bool hit = tn <= fminf(tf, cull);
-            if (hit)
-            {
-                if (imask & (1u << s))
-                {
-                    inner_hits |= 1u << (s ^ r.xkey);
-                }
-                else
-                {
-                    const uint32_t m = ((s < 4 ? meta_lo : meta_hi) >> (8 * (s & 3))) & 0xFFu;
-                    // 0xFF marks an empty slot (its inverted box can still
-                    // pass the padded test on a tiny node)
-                    if (m != 0xFFu)
-                    {
-                        const uint32_t off = m >> 2, cnt = (m & 3u) + 1u;
-                        trimask |= ((1u << cnt) - 1u) << off;
-                    }
-                }
-            }
-        }
-        st.grp_base = child_base;
-        st.grp_mask = inner_hits;
+        trimask = h.trimask;
+        tri_base = h.tri_base;
+        st.grp_base = h.child_base;
+        st.grp_mask = h.inner;
         st.visit = false;
     }
     // ---- triangles of the node just visited
@@ -294,50 +384,50 @@ WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
         const uint32_t i = __builtin_ctz(trimask);
         trimask &= trimask - 1u;
         WL_TRAV_CHECK((size_t)tri_base + i < wl_trav_tris_bound, "triangle index");
-        const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
-        const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
         if (COUNT)
             ++n_tris;
-        float t, b1, b2;
-        if (tri_test(r, w0, w1, w2, t, b1, b2))
-        {
-            const uint32_t prim = wl_f2u(w2.y);
-            if (ANY || t < st.res.t || (t == st.res.t && prim < st.res.prim))
-            {
-                st.res.t = t;
-                st.res.prim = prim;
-                st.res.b1 = b1;
-                st.res.b2 = b2;
-                if (AUX)
-                {
-                    st.res.input = wl_f2u(w2.z);
-                    st.res.local = wl_f2u(w2.w);
-                }
-                if (ANY)
-                    return true;
-            }
-        }
+        if (tri_visit<ANY, AUX>(tris, tri_base, i, r, st) && ANY)
+            return true;
     }
     // ---- next node: from the current group, else from the stack
-    if (st.grp_mask == 0)
-    {
-        if (st.sp == 0)
-            return true;
-        --st.sp;
-        const uint32_t e = lds_stack_lane[st.sp * BLOCK];
-        st.grp_base = (e >> 8) << 3;
-        st.grp_mask = e & 0xFFu;
-    }
-    const uint32_t pos = __builtin_ctz(st.grp_mask);
-    st.grp_mask &= st.grp_mask - 1u;
-    st.node = st.grp_base + (pos ^ r.xkey);
-    if (st.grp_mask)
-    {
-        WL_TRAV_CHECK(st.sp < WL_BVH8_STACK_DEPTH, "stack depth");
-        lds_stack_lane[st.sp * BLOCK] = ((st.grp_base >> 3) << 8) | st.grp_mask;
-        ++st.sp;
-    }
+    WL_TRAV_CHECK((st.grp_mask & (st.grp_mask - 1u)) == 0u || st.sp < WL_BVH8_STACK_DEPTH,
+                  "stack depth");
+    if (!next_node<BLOCK>(r, st, lds_stack_lane))
+        return true;
     st.visit = true;
+    return false;
+}
+
+// Closest-hit step: one pending triangle, or else one node (the next from
+// the group or the stack, then its box test).  Returns true when the ray is
+// finished.
+template <int BLOCK, bool AUX = false>
+WL_HD bool trav_step_one(const WlBvh8Node *__restrict__ nodes,
+                         const WlTriRecord *__restrict__ tris, const TravRay &r,
+                         TravState &st, uint32_t *lds_stack_lane)
+{
+    if (st.trimask)
+    {
+        const uint32_t i = __builtin_ctz(st.trimask);
+        st.trimask &= st.trimask - 1u;
+        WL_TRAV_CHECK((size_t)st.tri_base + i < wl_trav_tris_bound, "triangle index");
+        tri_visit<false, AUX>(tris, st.tri_base, i, r, st);
+        return false;
+    }
+    if (!st.visit)
+    {
+        WL_TRAV_CHECK((st.grp_mask & (st.grp_mask - 1u)) == 0u || st.sp < WL_BVH8_STACK_DEPTH,
+                  "stack depth");
+        if (!next_node<BLOCK>(r, st, lds_stack_lane))
+            return true;
+    }
+    WL_TRAV_CHECK(st.node < wl_trav_nodes_bound, "node index");
+    const NodeHits h = node_visit(nodes, st.node, r, st.res.t);
+    st.trimask = h.trimask;
+    st.tri_base = h.tri_base;
+    st.grp_base = h.child_base;
+    st.grp_mask = h.inner;
+    st.visit = false;
     return false;
 }
 

@@ -3,8 +3,10 @@
 // Incoherent rays finish after very different numbers of steps, so a wave
 // that runs one ray per lane to completion idles most of its lanes near the
 // end.  Here a fixed grid of waves loops: every lane advances its ray by one
-// node (trav_step); once at least kRefillLanes lanes of a wave are idle, the
-// idle lanes start the next rays of the wave's own contiguous range of ray
+// step (any-hit: a node and its triangles, trav_step; closest hit: one node
+// or one triangle, trav_step_one); once at least kRefillLanes lanes of a
+// wave are idle, the idle lanes start the next rays of the wave's own
+// contiguous range of ray
 // indices (after Aila & Laine, "Understanding the Efficiency of Ray Traversal
 // on GPUs", HPG 2009).
 //
@@ -81,7 +83,15 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                 break;
             continue;
         }
-        if (alive && trav_step<ANY, false, BLOCK, AUX>(ln, lt, r, st, stack_lane, nn, nt))
+        bool fin = false;
+        if (alive)
+        {
+            if constexpr (ANY)
+                fin = trav_step<ANY, false, BLOCK, AUX>(ln, lt, r, st, stack_lane, nn, nt);
+            else
+                fin = trav_step_one<BLOCK, AUX>(ln, lt, r, st, stack_lane);
+        }
+        if (fin)
         {
             trav_finish(st.res);
             emit(rid, st.res);
