@@ -19,6 +19,11 @@ namespace
 {
 
 constexpr int kBins = 32;
+// SAH cost of a traversal step relative to one triangle test.  Measured on
+// the 2M-triangle soup (closest hit steps one node or one triangle): 0.7
+// keeps incoherent closest hit level with 1.0 and makes the path tracer's
+// extension and shadow rays 2 % faster (fewer triangle tests per ray).
+constexpr double kSahTraversal = 0.7;
 constexpr uint32_t kParallelMin = 1u << 16;
 
 struct Aabb
@@ -131,7 +136,7 @@ struct Bvh2Builder
                         if (c == 0 || rcnt[b + 1] == 0)
                             continue;
                         double cost =
-                            1.0 + (acc.area() * c + rarea[b + 1] * rcnt[b + 1]) /
+                            kSahTraversal + (acc.area() * c + rarea[b + 1] * rcnt[b + 1]) /
                                       (parea > 0 ? parea : 1.0);
                         if (cost < best_cost)
                         {
