@@ -99,15 +99,8 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
     const uint32_t *qreq = parity ? B.qreq1 : B.qreq0; // multi-request batches
-    auto load = [&](uint32_t i, TravRay &r) {
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-        const float4 a = rp[0], b = rp[1];
-        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-    };
-    auto store = [&](uint32_t i, const TravResult &h) {
-        reinterpret_cast<float4 *>(B.hits)[i] =
-            make_float4(h.t, __uint_as_float(h.prim), h.b1, h.b2);
-    };
+    auto load = [&](uint32_t i, TravRay &r) { load_ray_nt(rays, i, r); };
+    auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
     if constexpr (MULTI)
         trav_persistent<false, kPtBlock, false, true>(
             nullptr, nullptr, count, stack + threadIdx.x,
@@ -627,11 +620,7 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
-    auto load = [&](uint32_t i, TravRay &r) {
-        const float4 *rp = reinterpret_cast<const float4 *>(B.srays + i);
-        const float4 a = rp[0], b = rp[1];
-        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
-    };
+    auto load = [&](uint32_t i, TravRay &r) { load_ray_nt(B.srays, i, r); };
     auto add = [&](uint32_t i, const TravResult &h) {
         if (h.prim != WL_INVALID)
             return;

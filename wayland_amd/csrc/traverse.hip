@@ -16,15 +16,9 @@ namespace wl
 
 constexpr int kTraceBlock = 256;
 
-// Rays are read once and hits written once: non-temporal accesses keep them
-// from displacing BVH nodes in the caches.
-typedef float f4v __attribute__((ext_vector_type(4)));
-
 __device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32_t i, TravRay &r)
 {
-    const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
-    const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
-    trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+    load_ray_nt(rays, i, r);
 }
 
 template <int STACK>
@@ -37,10 +31,7 @@ void k_trace_closest_p(
     trav_persistent<false, kTraceBlock, false>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
-        [&](uint32_t i, const TravResult &h) {
-            const f4v v = { h.t, __uint_as_float(h.prim), h.b1, h.b2 };
-            __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(hits) + i);
-        });
+        [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); });
 }
 
 template <int STACK>

@@ -16,10 +16,30 @@
 // the loop when its range is used up and none of its lanes holds a ray, so
 // the grid always drains.
 #pragma once
+#include "../../include/wayland_hip.h"
 #include "traverse_device.h"
 
 namespace wl
 {
+
+// Ray and hit streams are read once and written once: non-temporal accesses
+// keep them from displacing BVH nodes in the caches.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load_ray_nt(const wl_ray *__restrict__ rays, uint32_t i,
+                                            TravRay &r)
+{
+    const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
+    const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
+    trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, a.w, b.w);
+}
+
+__device__ __forceinline__ void store_hit_nt(wl_hit *__restrict__ hits, uint32_t i,
+                                             const TravResult &h)
+{
+    const f4v v = { h.t, __uint_as_float(h.prim), h.b1, h.b2 };
+    __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(hits) + i);
+}
 
 constexpr int kRefillLanes = 16;
 
