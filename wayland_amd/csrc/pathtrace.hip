@@ -328,7 +328,6 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
             const float4 s_pos = B.pos[path], s_dir = B.dir[path], s_thr = B.thr[path];
             const float4 s_ltt = B.ltt[path];
-            float4 s_rad = B.rad[path];
             V3 rayPos = v3of(s_pos);
             float lastPdf = s_pos.w;
             V3 rayDir = v3of(s_dir);
@@ -336,11 +335,16 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             V3 throughput = v3of(s_thr);
             uint32_t depth = __float_as_uint(s_thr.w);
             V3 lastTraceTerm = v3of(s_ltt);
-            V3 radiance = v3of(s_rad);
+            // radiance changes only at a miss or an emitter hit: read and
+            // written back only then (or when guide training needs it)
+            const bool gtrain = GTRAIN && (P.gmode & 2u);
+            V3 radiance = mk(0, 0, 0);
+            bool rad_dirty = false;
+            if (gtrain)
+                radiance = v3of(B.rad[path]);
             const uint32_t prim = __float_as_uint(hit.y);
             bool done = false;
             // guiding: training record state of the path
-            const bool gtrain = GTRAIN && (P.gmode & 2u);
             float4 gacc = make_float4(0.f, 0.f, 0.f, 0.f);
             if (gtrain)
                 gacc = B.gacc[path];
@@ -350,7 +354,10 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (prim == WL_INVALID)
             {
                 // __miss__radiance: radiance += throughput * bg
+                if (!gtrain)
+                    radiance = v3of(B.rad[path]);
                 radiance = radiance + throughput * ld3(P.bg);
+                rad_dirty = true;
                 done = true;
             }
             else
@@ -393,6 +400,9 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                             neePdf = pdf * dist * dist / fabsf(dot(Ns, rayDir));
                         }
                         const float uptPdf = lastPdf;
+                        if (!gtrain)
+                            radiance = v3of(B.rad[path]);
+                        rad_dirty = true;
                         radiance = radiance + throughput * ld3(lt.L) * lastTraceTerm *
                                                   (uptPdf / (uptPdf + neePdf));
                     }
@@ -563,7 +573,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     }
                 }
             }
-            B.rad[path] = f4(radiance, 0.0f);
+            if (rad_dirty)
+                B.rad[path] = f4(radiance, 0.0f);
             if (gtrain)
             {
                 // radiance reaching the path after its first guided vertex
