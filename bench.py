@@ -108,10 +108,13 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
     return out
 
 
-def relmse(img, ref, trim=0.0):
+def relmse(img, ref, trim=0.0, mask=None):
     """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01); trim > 0
-    drops that fraction of the largest per-pixel errors (fireflies)."""
+    drops that fraction of the largest per-pixel errors (fireflies); mask
+    keeps only the pixels where it is true."""
     e = ((img - ref) ** 2 / (ref ** 2 + 0.01)).reshape(-1, 3).mean(1)
+    if mask is not None:
+        e = e[np.asarray(mask).reshape(-1)]
     if trim > 0:
         e = np.sort(e)[: e.size - int(e.size * trim)]
     return float(e.mean())
@@ -149,6 +152,25 @@ def bench_guide(args, ctx, wl, torch):
         torch.cuda.synchronize()
         return pt.resolve(f, spp).cpu().numpy(), time.perf_counter() - t
 
+    # pixels whose camera ray (through the pixel centre) first meets the metal
+    # floor, where the guided vertex is; the glass sphere's pixels carry the
+    # glass-only paths that no guided vertex lies on
+    cam = sc.camera
+    c = (np.arange(res, dtype=np.float64) + 0.5) / res * 2 - 1
+    U, V = np.meshgrid(c, c)  # row = y, pixel index = y * res + x
+    d = (np.asarray(cam.lookat, np.float64)[None, None] + U[..., None] * np.asarray(cam.right)
+         + V[..., None] * np.asarray(cam.up))
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    prays = np.zeros((res * res, 8), np.float32)
+    prays[:, 0:3] = cam.pos
+    prays[:, 3] = 1e-3
+    prays[:, 4:7] = d.reshape(-1, 3)
+    prays[:, 7] = 1e30
+    prim = wl.hits_to_numpy(accel.trace_closest(torch.from_numpy(prays).cuda()))["prim"]
+    flat = sc.flatten()
+    f0, fn = int(flat["mesh_tri_offset"][1]), int(flat["mesh_tri_count"][1])
+    floor = (prim >= f0) & (prim < f0 + fn)
+
     # allocate the per-path guide buffers outside the timed runs
     warm_guide = wl.Guide(ctx)
     pt.set_guide(warm_guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
@@ -157,7 +179,7 @@ def bench_guide(args, ctx, wl, torch):
     warm_guide.close()
     # independent runs: fresh network each time, training inside the timing
     runs = args.guide_runs
-    g_rel, g_trim, g_t, steps = [], [], [], 0
+    g_rel, g_trim, g_floor, g_t, steps = [], [], [], [], 0
     for r in range(runs):
         guide = wl.Guide(ctx, seed=r + 1)
         pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
@@ -167,23 +189,28 @@ def bench_guide(args, ctx, wl, torch):
         guide.close()
         g_rel.append(relmse(img_g, ref))
         g_trim.append(relmse(img_g, ref, 1e-3))
+        g_floor.append(relmse(img_g, ref, mask=floor))
         g_t.append(t_g)
     # plain PT per-sample time, then the same wall time as the guided runs
     _, t_probe = progressive(8, 1 << 26)
     spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 8))))
-    p_rel, p_trim, p_t = [], [], []
+    p_rel, p_trim, p_floor, p_t = [], [], [], []
     for r in range(runs):
         img_p, t_p = progressive(spp_p, (1 << 27) + (r << 20))
         p_rel.append(relmse(img_p, ref))
         p_trim.append(relmse(img_p, ref, 1e-3))
+        p_floor.append(relmse(img_p, ref, mask=floor))
         p_t.append(t_p)
     out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
            "reference": f"plain PT {ref_spp} spp on the GPU",
            "runs": runs,
+           "floor_pixels": int(floor.sum()),
            "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)),
-                      "relmse": g_rel, "relmse_trim_0.1pct": g_trim, "adam_steps": steps},
+                      "relmse": g_rel, "relmse_trim_0.1pct": g_trim, "relmse_floor": g_floor,
+                      "adam_steps": steps},
            "plain": {"spp": spp_p, "seconds": float(np.mean(p_t)), "relmse": p_rel,
-                     "relmse_trim_0.1pct": p_trim},
+                     "relmse_trim_0.1pct": p_trim, "relmse_floor": p_floor},
+           "relmse_floor_ratio_equal_time": float(np.mean(g_floor) / np.mean(p_floor)),
            "relmse_ratio_equal_time": float(np.mean(g_rel) / np.mean(p_rel)),
            "relmse_median_ratio_equal_time": float(np.median(g_rel) / np.median(p_rel)),
            "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim))}
