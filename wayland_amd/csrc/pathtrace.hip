@@ -48,6 +48,57 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
     return base + prefix;
 }
 
+// ---- workgroup-level append to two queues -----------------------------------
+// Every wave of the workgroup calls this in the same loop pass.  Per wave a
+// ballot and mbcnt prefix; per workgroup one atomic per queue, issued by one
+// thread.  All waves appending to one counter serialise at one L2 channel,
+// so a per-wave atomic makes the counter the bottleneck of the shade kernel;
+// a per-workgroup one divides that by the waves per workgroup.  The LDS
+// slots alternate between passes (pass parity), so two barriers per pass
+// suffice.
+struct BlockAppendLds
+{
+    uint32_t cnt[2][2][kPtBlock / 64]; // [pass parity][queue][wave]
+    uint32_t base[2][2];               // [pass parity][queue]
+};
+
+__device__ __forceinline__ void block_append2(BlockAppendLds &L, uint32_t pass,
+                                              uint32_t *c0, bool p0, uint32_t *c1, bool p1,
+                                              uint32_t &slot0, uint32_t &slot1)
+{
+    const uint32_t par = pass & 1u;
+    const uint32_t wave = threadIdx.x / 64u, lane = __lane_id();
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+    if (lane == 0)
+    {
+        L.cnt[par][0][wave] = (uint32_t)__popcll(m0);
+        L.cnt[par][1][wave] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t t0 = 0, t1 = 0;
+        for (uint32_t w = 0; w < kPtBlock / 64; ++w)
+        {
+            t0 += L.cnt[par][0][w];
+            t1 += L.cnt[par][1][w];
+        }
+        L.base[par][0] = t0 ? atomicAdd(c0, t0) : 0u;
+        L.base[par][1] = t1 ? atomicAdd(c1, t1) : 0u;
+    }
+    __syncthreads();
+    uint32_t o0 = L.base[par][0], o1 = L.base[par][1];
+    for (uint32_t w = 0; w < wave; ++w)
+    {
+        o0 += L.cnt[par][0][w];
+        o1 += L.cnt[par][1][w];
+    }
+    slot0 = o0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0u));
+    slot1 = o1 + __builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0u));
+}
+
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3of(float4 a) { return mk(a.x, a.y, a.z); }
 
@@ -307,6 +358,10 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
     uint32_t *qout = parity ? B.qpath0 : B.qpath1;
     wl_ray *rout = parity ? B.rays0 : B.rays1;
+    __shared__ BlockAppendLds app;
+    uint32_t pass = 0;
+    // every thread of the workgroup runs the same passes (base is per
+    // workgroup), as the workgroup barriers of the append require
     for (uint32_t base = blockIdx.x * kPtBlock; base < count; base += gridDim.x * kPtBlock)
     {
         const uint32_t j = base + threadIdx.x;
@@ -602,7 +657,9 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             }
         }
         // ---- compaction (whole wave participates)
-        const uint32_t slot = wave_append(&B.counters[parity ^ 1], cont);
+        uint32_t slot, sslot;
+        block_append2(app, pass++, &B.counters[parity ^ 1], cont, &B.counters[2], shadow, slot,
+                      sslot);
         if (cont)
         {
             qout[slot] = path;
@@ -610,7 +667,6 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (MULTI)
                 (parity ? B.qreq0 : B.qreq1)[slot] = req;
         }
-        const uint32_t sslot = wave_append(&B.counters[2], shadow);
         if (shadow)
         {
             B.spath[sslot] = path;
