@@ -191,9 +191,11 @@ def bench_guide(args, ctx, wl, torch):
         g_trim.append(relmse(img_g, ref, 1e-3))
         g_floor.append(relmse(img_g, ref, mask=floor))
         g_t.append(t_g)
-    # plain PT per-sample time, then the same wall time as the guided runs
-    _, t_probe = progressive(8, 1 << 26)
-    spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 8))))
+    # plain PT per-sample time (after a warm pass), then the same wall time as
+    # the guided runs
+    progressive(2, (1 << 26) + 64)
+    _, t_probe = progressive(16, 1 << 26)
+    spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 16))))
     p_rel, p_trim, p_floor, p_t = [], [], [], []
     for r in range(runs):
         img_p, t_p = progressive(spp_p, (1 << 27) + (r << 20))
