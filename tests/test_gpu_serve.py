@@ -82,7 +82,7 @@ def test_serve_autotune_then_render(gpu_ctx):
     server = wl.Server(gpu_ctx)
     cfg = server.autotune(reqs, max_spp=2)
     print(f"chosen {cfg}")
-    assert cfg["batch_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25)
+    assert cfg["batch_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert cfg["shade_blocks_per_cu"] in (4, 8, 16)
     films, _, _ = server.render(reqs[:3])
     for k in range(3):

@@ -463,7 +463,8 @@ wl_status wl_server_autotune(wl_server s, const wl_render_request *reqs, uint32_
     const wl_serve_config saved = s->cfg;
     wl_serve_config best = saved;
     double best_rate = -1.0;
-    const uint32_t batches[] = { 1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25 };
+    const uint32_t batches[] = { 1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25,
+                                 1u << 26 };
     const uint32_t shades[] = { 4, 8, 16 };
     wl_status st = WL_SUCCESS;
     for (uint32_t bp : batches)
