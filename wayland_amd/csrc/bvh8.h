@@ -13,11 +13,14 @@
 //   bytes  20..23  tri_base    first triangle record of this node's leaves
 //   bytes  24..31  meta[8]     leaf slot: (offset << 2) | (count - 1);
 //                              empty slot: 0xFF
-//   bytes  32..127 q[6][8]     u16 quantised child boxes: lo_x, lo_y, lo_z,
-//                              hi_x, hi_y, hi_z; plane = p + scale * q.
+//   bytes  32..127 q[6][8]     quantised child boxes: lo_x, lo_y, lo_z,
+//                              hi_x, hi_y, hi_z; plane = p + scale * q, q an
+//                              integer 0..2047 held as binary16 (exact), so
+//                              gfx950's v_fma_mix_f32 reads it directly.
 // Child boxes are rounded outward and padded by one quantum, so the decoded
-// box always contains the exact child box.  Empty slots have lo = 0xFFFF and
-// hi = 0, which a near/far-plane slab test never hits.
+// box always contains the exact child box.  Empty slots have lo = 65504 (the
+// largest finite binary16) and hi = 0, which a near/far-plane slab test never
+// hits.
 //
 // Inner children are addressed as child_base + slot: every node that has an
 // inner child owns a group of eight node slots.  That costs node memory for

@@ -197,21 +197,41 @@ struct Bvh2Builder
 
 inline float exp2i(int e) { return std::ldexp(1.0f, e); }
 
-// Per-axis exponent so that (hi - p) / 2^e <= 65533.
+// Quantised planes are integers 0..kQMax, stored as binary16 (exact for
+// integers up to 2048): the traversal's fma then takes them as a
+// mixed-precision operand with no separate conversion.
+constexpr int kQMax = 2047;
+
+// Per-axis exponent so that (hi - p) / 2^e <= kQMax - 2 (the outward
+// rounding and one quantum of padding on either side then stay in range).
 int choose_exponent(float lo, float hi)
 {
+    const double room = kQMax - 2;
     double ext = (double)hi - (double)lo;
     if (!(ext > 0))
         return -120;
     int x;
-    std::frexp(ext / 65533.0, &x); // ext/65533 = m * 2^x, m in [0.5, 1)
+    std::frexp(ext / room, &x); // ext/room = m * 2^x, m in [0.5, 1)
     int e = x;
-    if (std::ldexp(1.0, e - 1) >= ext / 65533.0)
+    if (std::ldexp(1.0, e - 1) >= ext / room)
         e -= 1;
-    while (std::ldexp(1.0, e) * 65533.0 < ext)
+    while (std::ldexp(1.0, e) * room < ext)
         ++e;
     return std::min(std::max(e, -120), 127);
 }
+
+// binary16 bits of an integer 0 <= n <= 2047 (exact: 11 significant bits).
+uint16_t half_of_int(int n)
+{
+    if (n <= 0)
+        return 0;
+    int e = 0;
+    while ((n >> (e + 1)) != 0)
+        ++e; // 2^e <= n < 2^(e+1), e <= 10
+    const uint32_t mant = ((uint32_t)n << (10 - e)) & 0x3FFu;
+    return (uint16_t)(((uint32_t)(e + 15) << 10) | mant);
+}
+constexpr uint16_t kHalfMaxFinite = 0x7BFF; // 65504: the empty slot's lo plane
 
 } // namespace
 
@@ -347,7 +367,7 @@ bool build_bvh8(const float *tris, uint32_t ntris, int nthreads, Bvh8Host *out,
             node.meta[s] = 0xFF;
             for (int k = 0; k < 3; ++k)
             {
-                node.q[k][s] = 0xFFFF;
+                node.q[k][s] = kHalfMaxFinite;
                 node.q[3 + k][s] = 0;
             }
         }
@@ -390,9 +410,9 @@ bool build_bvh8(const float *tris, uint32_t ntris, int nthreads, Bvh8Host *out,
                 double qlo = std::floor(((double)k.box.lo[a] - node.p[a]) / scale[a]) - 1.0;
                 double qhi = std::ceil(((double)k.box.hi[a] - node.p[a]) / scale[a]) + 1.0;
                 qlo = std::max(qlo, 0.0);
-                qhi = std::min(qhi, 65535.0);
-                node.q[a][s] = (uint16_t)qlo;
-                node.q[3 + a][s] = (uint16_t)qhi;
+                qhi = std::min(qhi, (double)kQMax);
+                node.q[a][s] = half_of_int((int)qlo);
+                node.q[3 + a][s] = half_of_int((int)qhi);
             }
             if (k.count == 0)
             {

@@ -48,8 +48,9 @@ struct TravRay
     float tmin, tmax;
     float inv[3];
     float Sx, Sy, Sz;
-    uint32_t kxyz;         // kx | ky << 2 | kz << 4 (the triangle test's axis order)
-    uint32_t neg;          // bit k: d_k < 0; slot ^ neg ^ 7 = visit order key (near first)
+    // bits 0-2: bit k set for d_k < 0 (slot ^ neg ^ 7 = visit order key, near
+    // first); bits 3-8: kx | ky << 2 | kz << 4 (the triangle test's axis order)
+    uint32_t code;
 };
 
 WL_HD float sel3(int k, float a, float b, float c)
@@ -81,7 +82,7 @@ WL_HD void trav_setup(TravRay &r, float ox, float oy,
         kx = ky;
         ky = t;
     }
-    r.kxyz = (uint32_t)kx | (uint32_t)ky << 2 | (uint32_t)kz << 4;
+    const uint32_t kxyz = (uint32_t)kx | (uint32_t)ky << 2 | (uint32_t)kz << 4;
     r.Sx = sel3(kx, dx, dy, dz) / dkz;
     r.Sy = sel3(ky, dx, dy, dz) / dkz;
     r.Sz = 1.0f / dkz;
@@ -102,17 +103,18 @@ WL_HD void trav_setup(TravRay &r, float ox, float oy,
         if (r.inv[i] < 0.0f)
             neg |= 1u << i;
     }
-    r.neg = neg;
+    r.code = neg | kxyz << 3;
 }
 
 // Watertight test; returns true and (t, b1, b2) for tmin <= t <= tmax.
 WL_HD bool tri_test(const TravRay &r, const float4 w0,
                                          const float4 w1, const float4 w2,
-                                         float &t_out, float &b1_out,
+                                         float tmax, float &t_out, float &b1_out,
                                          float &b2_out)
 {
     // A = v0 - o, B = v1 - o, C = v2 - o, components permuted (kx, ky, kz)
-    const int kx = (int)(r.kxyz & 3u), ky = (int)((r.kxyz >> 2) & 3u), kz = (int)(r.kxyz >> 4);
+    const int kx = (int)((r.code >> 3) & 3u), ky = (int)((r.code >> 5) & 3u),
+              kz = (int)((r.code >> 7) & 3u);
     const float okx = sel3(kx, r.o[0], r.o[1], r.o[2]);
     const float oky = sel3(ky, r.o[0], r.o[1], r.o[2]);
     const float okz = sel3(kz, r.o[0], r.o[1], r.o[2]);
@@ -154,7 +156,7 @@ WL_HD bool tri_test(const TravRay &r, const float4 w0,
     const float T = U * Az + V * Bz + W * Cz;
     const float rcp = 1.0f / det;
     const float t = T * rcp;
-    if (!(t >= r.tmin && t <= r.tmax))
+    if (!(t >= r.tmin && t <= tmax))
         return false;
     t_out = t;
     b1_out = V * rcp;
@@ -162,8 +164,16 @@ WL_HD bool tri_test(const TravRay &r, const float4 w0,
     return true;
 }
 
-WL_HD float q_lo16(uint32_t w) { return (float)(w & 0xFFFFu); }
-WL_HD float q_hi16(uint32_t w) { return (float)(w >> 16); }
+// Quantised planes are binary16 halves of a word; the fma that consumes the
+// widened value compiles to one v_fma_mix_f32 (conversion folded in).
+WL_HD float q_lo16(uint32_t w)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+}
+WL_HD float q_hi16(uint32_t w)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
 
 struct TravResult
 {
@@ -215,8 +225,6 @@ WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
     const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
     const uint4 h0 = np[0];
     const uint4 h1 = np[1];
-    const uint4 qlx = np[2], qly = np[3], qlz = np[4];
-    const uint4 qhx = np[5], qhy = np[6], qhz = np[7];
     const float px = wl_u2f(h0.x), py = wl_u2f(h0.y), pz = wl_u2f(h0.z);
     const uint32_t ew = h0.w;
     const float sx = wl_u2f((ew & 0xFFu) << 23);
@@ -230,10 +238,11 @@ WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
     const float az = (pz - r.o[2]) * r.inv[2];
     const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
     // near/far quantised planes chosen by the ray's direction signs
-    const bool nx = r.neg & 1u, ny = r.neg & 2u, nz = r.neg & 4u;
-    const uint4 nqx = nx ? qhx : qlx, fqx = nx ? qlx : qhx;
-    const uint4 nqy = ny ? qhy : qly, fqy = ny ? qly : qhy;
-    const uint4 nqz = nz ? qhz : qlz, fqz = nz ? qlz : qhz;
+    // (lo planes at words 2-4, hi planes at 5-7: a per-lane offset picks them)
+    const uint32_t nx = r.code & 1u, ny = (r.code >> 1) & 1u, nz = (r.code >> 2) & 1u;
+    const uint4 nqx = np[2 + 3 * nx], fqx = np[5 - 3 * nx];
+    const uint4 nqy = np[3 + 3 * ny], fqy = np[6 - 3 * ny];
+    const uint4 nqz = np[4 + 3 * nz], fqz = np[7 - 3 * nz];
     const float cull = t_best * 1.00001f;
     const float tmin = r.tmin;
     const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
@@ -278,7 +287,7 @@ WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
     // inner hits to key space: bit s -> bit s ^ k, k = neg ^ 7 (xor of a 3-bit index
     // = swaps of single bits, pairs and nibbles)
     uint32_t m = hitmask & imask;
-    const uint32_t k = r.neg ^ 7u;
+    const uint32_t k = (r.code & 7u) ^ 7u;
     m = (k & 1u) ? (((m & 0x55u) << 1) | ((m >> 1) & 0x55u)) : m;
     m = (k & 2u) ? (((m & 0x33u) << 2) | ((m >> 2) & 0x33u)) : m;
     m = (k & 4u) ? (((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu)) : m;
@@ -312,7 +321,9 @@ WL_HD bool tri_visit(const WlTriRecord *__restrict__ tris, uint32_t tri_base, ui
     const float4 *tp = reinterpret_cast<const float4 *>(tris + tri_base + i);
     const float4 w0 = tp[0], w1 = tp[1], w2 = tp[2];
     float t, b1, b2;
-    if (tri_test(r, w0, w1, w2, t, b1, b2))
+    // the best hit so far bounds t: closest hit keeps only t <= res.t (ties to
+    // the smaller primitive id below); any-hit holds res.t = tmax until its hit
+    if (tri_test(r, w0, w1, w2, st.res.t, t, b1, b2))
     {
         const uint32_t prim = wl_f2u(w2.y);
         if (ANY || t < st.res.t || (t == st.res.t && prim < st.res.prim))
@@ -347,7 +358,7 @@ WL_HD bool next_node(const TravRay &r, TravState &st, uint32_t *lds_stack_lane)
     }
     const uint32_t pos = __builtin_ctz(st.grp_mask);
     st.grp_mask &= st.grp_mask - 1u;
-    st.node = st.grp_base + (pos ^ r.neg ^ 7u);
+    st.node = st.grp_base + (pos ^ (r.code & 7u) ^ 7u);
     if (st.grp_mask)
     {
         lds_stack_lane[st.sp * BLOCK] = ((st.grp_base >> 3) << 8) | st.grp_mask;
