@@ -16,7 +16,7 @@
 // Two step shapes share the node and triangle code.  trav_step visits a node
 // and then tests all of its triangles (any-hit, which stops at the first hit,
 // and the instrumented kernel).  The closest-hit kernels step with
-// trav_node_one / trav_tri_one instead: a lane either visits one node or
+// trav_step_one instead: a lane either visits one node or
 // tests one pending triangle per step, so lanes with several triangles do not
 // hold the whole wave in a triangle loop.  Both visit the same nodes and test
 // the same triangles in the same order, so the results are the same.
@@ -115,18 +115,13 @@ WL_HD bool tri_test(const TravRay &r, const float4 w0,
     // A = v0 - o, B = v1 - o, C = v2 - o, components permuted (kx, ky, kz)
     const int kx = (int)((r.code >> 3) & 3u), ky = (int)((r.code >> 5) & 3u),
               kz = (int)((r.code >> 7) & 3u);
-    const float okx = sel3(kx, r.o[0], r.o[1], r.o[2]);
-    const float oky = sel3(ky, r.o[0], r.o[1], r.o[2]);
-    const float okz = sel3(kz, r.o[0], r.o[1], r.o[2]);
-    const float Akx = sel3(kx, w0.x, w0.y, w0.z) - okx;
-    const float Aky = sel3(ky, w0.x, w0.y, w0.z) - oky;
-    const float Akz = sel3(kz, w0.x, w0.y, w0.z) - okz;
-    const float Bkx = sel3(kx, w0.w, w1.x, w1.y) - okx;
-    const float Bky = sel3(ky, w0.w, w1.x, w1.y) - oky;
-    const float Bkz = sel3(kz, w0.w, w1.x, w1.y) - okz;
-    const float Ckx = sel3(kx, w1.z, w1.w, w2.x) - okx;
-    const float Cky = sel3(ky, w1.z, w1.w, w2.x) - oky;
-    const float Ckz = sel3(kz, w1.z, w1.w, w2.x) - okz;
+    // subtract in world axes, then permute (the same differences, fewer selects)
+    const float A0 = w0.x - r.o[0], A1 = w0.y - r.o[1], A2 = w0.z - r.o[2];
+    const float B0 = w0.w - r.o[0], B1 = w1.x - r.o[1], B2 = w1.y - r.o[2];
+    const float C0 = w1.z - r.o[0], C1 = w1.w - r.o[1], C2 = w2.x - r.o[2];
+    const float Akx = sel3(kx, A0, A1, A2), Aky = sel3(ky, A0, A1, A2), Akz = sel3(kz, A0, A1, A2);
+    const float Bkx = sel3(kx, B0, B1, B2), Bky = sel3(ky, B0, B1, B2), Bkz = sel3(kz, B0, B1, B2);
+    const float Ckx = sel3(kx, C0, C1, C2), Cky = sel3(ky, C0, C1, C2), Ckz = sel3(kz, C0, C1, C2);
     const float Ax = Akx - r.Sx * Akz;
     const float Ay = Aky - r.Sy * Akz;
     const float Bx = Bkx - r.Sx * Bkz;
@@ -192,7 +187,7 @@ struct TravState
     uint32_t grp_base, grp_mask, node;
     int sp;
     bool visit;
-    uint32_t trimask, tri_base; // pending triangles (trav_node_one / trav_tri_one)
+    uint32_t trimask, tri_base; // pending triangles (trav_step_one)
 };
 
 WL_HD void trav_init(TravState &st, const TravRay &r, bool active)
