@@ -152,13 +152,11 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     const uint32_t *qreq = parity ? B.qreq1 : B.qreq0; // multi-request batches
     auto load = [&](uint32_t i, TravRay &r) { load_ray_nt(rays, i, r); };
     auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
-    if constexpr (MULTI)
-        trav_persistent<false, kPtBlock, false, true>(
-            nullptr, nullptr, count, stack + threadIdx.x,
-            [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
-                const PtReq &R = reqs[qreq[i]];
-                n = R.S.nodes;
-                t = R.S.tris;
+    if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
+        trav_persistent<false, kPtBlock, false, true, false, 8>(
+            S.nodes, S.tris, count, stack + threadIdx.x,
+            [&](uint32_t i, TravRay &r, uint32_t &root) {
+                root = reqs[qreq[i]].root;
                 load(i, r);
             },
             store);
@@ -369,7 +367,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
         // material-sorted order: the j-th entry of the sorted queue
         const uint32_t i = (!MULTI && P0.sort_bins && in_range) ? B.sort_perm[j] : j;
         bool cont = false, shadow = false, shadow_gacc = false;
-        uint32_t path = 0, req = 0;
+        uint32_t path = 0, req = 0, root = 0;
         wl_ray next_ray{}, shadow_ray{};
         V3 contrib = mk(0, 0, 0);
         if (in_range)
@@ -378,6 +376,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (MULTI)
                 req = (parity ? B.qreq1 : B.qreq0)[i];
             const PtReq *rq = MULTI ? reqs + req : nullptr;
+            if (MULTI)
+                root = rq->root;
             const PtParams &P = MULTI ? rq->P : P0;
             const PtScene &S = MULTI ? rq->S : S0;
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
@@ -671,7 +671,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
         {
             B.spath[sslot] = path;
             if (MULTI)
-                B.sreq[sslot] = req;
+                B.sreq[sslot] = root;
             B.srays[sslot] = shadow_ray;
             B.scontrib[sslot] = f4(contrib, shadow_gacc ? 1.0f : 0.0f);
         }
@@ -708,13 +708,11 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
             B.gacc[path] = ga;
         }
     };
-    if constexpr (MULTI)
-        trav_persistent<true, kPtBlock, false, true>(
-            nullptr, nullptr, count, stack + threadIdx.x,
-            [&](uint32_t i, TravRay &r, const WlBvh8Node *&n, const WlTriRecord *&t) {
-                const PtReq &R = reqs[B.sreq[i]];
-                n = R.S.nodes;
-                t = R.S.tris;
+    if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
+        trav_persistent<true, kPtBlock, false, true, false, 8>(
+            S.nodes, S.tris, count, stack + threadIdx.x,
+            [&](uint32_t i, TravRay &r, uint32_t &root) {
+                root = B.sreq[i];
                 load(i, r);
             },
             add);
@@ -1051,28 +1049,55 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
         std::min<uint32_t>(blocks_all, cus * std::max<uint32_t>(1, L.shade_blocks_per_cu));
     const PtParams P0{};
     const PtScene S0{};
+    PtScene SA{}; // traversal: the node/triangle arena of the requests' BVHs
+    SA.nodes = L.nodes;
+    SA.tris = L.tris;
     hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, L);
     hipLaunchKernelGGL(k_raygen_multi, dim3(g_gen), dim3(kPtBlock), 0, st, L, B);
     for (uint32_t b = 0; b < L.max_depth; ++b)
     {
         const int parity = (int)(b & 1u);
         if (L.stack16)
-            hipLaunchKernelGGL((k_extend<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, S0, B,
+            hipLaunchKernelGGL((k_extend<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, SA, B,
                                parity, L.reqs);
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
-                               dim3(kPtBlock), 0, st, S0, B, parity, L.reqs);
+                               dim3(kPtBlock), 0, st, SA, B, parity, L.reqs);
         hipLaunchKernelGGL((k_shade<false, false, true>), dim3(g_shade), dim3(kPtBlock), 0, st,
                            P0, S0, B, parity, L.reqs);
         if (L.stack16)
-            hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, S0, B,
+            hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, SA, B,
                                L.reqs);
         else
             hipLaunchKernelGGL((k_shadow<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
-                               dim3(kPtBlock), 0, st, S0, B, L.reqs);
+                               dim3(kPtBlock), 0, st, SA, B, L.reqs);
     }
     hipLaunchKernelGGL(k_accumulate_multi, dim3(g_gen), dim3(kPtBlock), 0, st, L, B);
     hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, L);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_arena_copy_nodes(WlBvh8Node *__restrict__ dst,
+                                                          const WlBvh8Node *__restrict__ src,
+                                                          uint32_t n, uint32_t node0,
+                                                          uint32_t tri0)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    WlBvh8Node v = src[i];
+    v.child_base += node0;
+    v.tri_base += tri0;
+    dst[node0 + i] = v;
+}
+
+hipError_t pt_arena_copy_nodes(WlBvh8Node *dst, const WlBvh8Node *src, uint32_t n,
+                               uint32_t node0, uint32_t tri0, hipStream_t st)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_arena_copy_nodes, dim3((n + 255) / 256), dim3(256), 0, st, dst, src, n,
+                       node0, tri0);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ void k_trace_closest_p(
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
-    trav_persistent<false, kTraceBlock, false>(
+    trav_persistent<false, kTraceBlock, false, false, true, 4>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); });
@@ -40,7 +40,7 @@ void k_trace_any_p(const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__re
                    const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
-    trav_persistent<true, kTraceBlock, false>(
+    trav_persistent<true, kTraceBlock, false, false, true>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {

@@ -309,6 +309,7 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     }
     a->num_nodes = (uint32_t)host.nodes.size();
     a->num_tris = total;
+    a->num_tri_records = (uint32_t)host.tris.size();
     a->h_tris = std::move(tris);
     wl_accel_info &inf = a->info;
     inf.num_triangles = total;

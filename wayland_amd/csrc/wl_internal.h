@@ -37,6 +37,7 @@ struct wl_accel_s
     int occ_closest = 0, occ_any = 0; // resident workgroups per CU (queried once)
     uint32_t num_nodes = 0;
     uint64_t num_tris = 0;
+    uint32_t num_tri_records = 0; // entries of d_tris (leaf triangle records)
     std::vector<uint32_t> input_offsets;
     std::vector<uint32_t> input_tri_counts;
     std::vector<uint32_t> input_num_vertices;

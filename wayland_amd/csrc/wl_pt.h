@@ -95,7 +95,8 @@ struct PtBuffers
     uint32_t *sort_keys, *sort_perm; // material sort: key and order per queue slot
     uint32_t *sort_bins;             // [0, 1024) counts, [1024, 2048) offsets
     uint32_t *qreq0, *qreq1, *sreq; // multi-request batches: request of each
-                                    // ray queue slot / shadow queue slot
+                                    // ray queue slot / BVH root (node arena
+                                    // index) of each shadow queue slot
     uint32_t *counters;       // [0], [1] active queues, [2] shadow queue
     unsigned long long *stats; // [0] extension rays, [1] shadow rays
     // learned guiding (only with PtParams::gmode != 0)
@@ -113,14 +114,17 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                         int num_cus, hipStream_t st);
 
 // ---- batched serving: one megabatch mixes paths of many requests ----------
-// A request's scene, parameters and film; the ray and shadow queues of a
-// multi-request batch carry each entry's request id (qreq0/qreq1/sreq).
+// A request's scene, parameters and film; the ray queues of a multi-request
+// batch carry each entry's request id (qreq0/qreq1), the shadow queue the
+// root of the request's BVH in the node arena (sreq), the one thing the
+// shadow kernel needs.
 struct PtReq
 {
     PtParams P; // pix0 / npix / nsamples / frame0 unused; gmode 0
-    PtScene S;
+    PtScene S;  // S.nodes / S.tris: the request's own BVH (traversal uses the arena)
     float *film; // 6 floats per pixel (Kahan sum + compensation)
-    uint32_t pad_[2];
+    uint32_t root; // the request's BVH root in the launch's node arena
+    uint32_t pad_;
 };
 
 // Work item: all pixels of request `req` for samples [frame0, frame0 +
@@ -149,6 +153,11 @@ struct PtMultiLaunch
     const PtBatchHeader *batches; // all batches' headers
     uint32_t *batch_index;        // device counter of the batch being run
     PtBatchHeader *header;        // the current batch's header
+    // the requests' BVHs back to back, child and triangle indices rebased, so
+    // multi-request traversal keeps the single-BVH kernel's registers and
+    // starts each ray at its request's root
+    const WlBvh8Node *nodes;
+    const WlTriRecord *tris;
     uint32_t max_depth;  // bounces enqueued (maximum over the requests)
     uint32_t stack16;    // every BVH at most 16 levels deep
     uint32_t capacity;   // paths the buffers hold
@@ -161,6 +170,11 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
                         hipStream_t st);
 // One-time occupancy queries of the multi-request kernels (outside capture).
 void pt_multi_init();
+// Copies n BVH nodes into an arena at node offset node0, adding node0 to the
+// child indices and tri0 to the triangle indices (the triangle records go to
+// tri0 unchanged).
+hipError_t pt_arena_copy_nodes(WlBvh8Node *dst, const WlBvh8Node *src, uint32_t n,
+                               uint32_t node0, uint32_t tri0, hipStream_t st);
 // Moves the finished batch's training records with a positive weight into
 // the training buffer.
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus,

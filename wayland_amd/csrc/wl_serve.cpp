@@ -43,9 +43,14 @@ struct wl_server_s
     uint32_t batches_cap = 0;
     uint32_t *d_batch_index = nullptr;
     wl::PtBatchHeader *d_header = nullptr;
-    // captured megabatch
+    // the requests' BVHs back to back (traversal arena), refilled every call
+    WlBvh8Node *d_arena_nodes = nullptr;
+    uint32_t arena_nodes_cap = 0;
+    WlTriRecord *d_arena_tris = nullptr;
+    uint32_t arena_tris_cap = 0;
+    // captured megabatch and everything it bakes in (compared field by field)
     hipGraphExec_t exec = nullptr;
-    uint64_t graph_key = 0;
+    std::vector<uint64_t> graph_key;
     std::vector<hipEvent_t> events;
 };
 
@@ -79,7 +84,7 @@ void drop_graph(wl_server s)
         hipGraphExecDestroy(s->exec);
         s->exec = nullptr;
     }
-    s->graph_key = 0;
+    s->graph_key.clear();
 }
 
 wl_status ensure_buffers(wl_server s, uint32_t capacity)
@@ -136,7 +141,9 @@ struct Plan
 };
 
 // FIFO packing: each megabatch takes, in request order, as many whole-image
-// sample passes of every unfinished request as still fit.
+// sample passes of every unfinished request as still fit.  (Items stay in
+// request order: grouping a megabatch's items by BVH was measured slower,
+// since the traversal's static per-wave ray ranges then get uneven work.)
 void plan(const wl_render_request *r, uint32_t n, uint32_t capacity, Plan &P)
 {
     std::vector<uint32_t> rem(n), next(n, 0);
@@ -216,6 +223,32 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
         max_depth = std::max(max_depth, r.max_depth);
         max_bvh = std::max(max_bvh, r.scene->accel->info.depth);
     }
+    // traversal arena: each distinct BVH once, its child and triangle indices
+    // rebased, so the multi-request traversal starts a ray at its request's
+    // root and otherwise runs the single-BVH kernel
+    std::vector<const wl_accel_s *> accs;
+    std::vector<uint32_t> acc_node0, acc_tri0;
+    uint64_t arena_nodes = 0, arena_tris = 0;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const wl_accel_s *a = reqs[k].scene->accel;
+        size_t j = 0;
+        while (j < accs.size() && accs[j] != a)
+            ++j;
+        if (j == accs.size())
+        {
+            accs.push_back(a);
+            acc_node0.push_back((uint32_t)arena_nodes);
+            acc_tri0.push_back((uint32_t)arena_tris);
+            arena_nodes += a->num_nodes;
+            arena_tris += std::max<uint32_t>(1, a->num_tri_records);
+            // stack entries hold child group indices / 8 in 24 bits
+            if (arena_nodes >= (1ull << 27) || arena_tris >= (1ull << 32))
+                return set_error(WL_ERROR_LIMIT_EXCEEDED,
+                                 "the requests' BVHs exceed the traversal arena (2^27 nodes)");
+        }
+        hreq[k].root = acc_node0[j];
+    }
     if (max_npix > cfg.batch_paths)
         return set_error(WL_ERROR_LIMIT_EXCEEDED,
                          "batch_paths must hold the largest request's pixel count");
@@ -229,6 +262,10 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
         st = grow(&s->d_items, s->items_cap, (uint32_t)pl.items.size());
     if (st == WL_SUCCESS)
         st = grow(&s->d_batches, s->batches_cap, nb);
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_arena_nodes, s->arena_nodes_cap, (uint32_t)arena_nodes);
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_arena_tris, s->arena_tris_cap, (uint32_t)arena_tris);
     if (st != WL_SUCCESS)
         return st;
     // the descriptors of this call; the graph reads them through fixed pointers
@@ -245,6 +282,8 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
     L.batches = s->d_batches;
     L.batch_index = s->d_batch_index;
     L.header = s->d_header;
+    L.nodes = s->d_arena_nodes;
+    L.tris = s->d_arena_tris;
     L.max_depth = max_depth;
     L.stack16 = max_bvh <= 16 ? 1u : 0u;
     L.capacity = s->capacity;
@@ -255,15 +294,32 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
     WL_HIP_CHECK(hipEventRecord(ready, user));
     WL_HIP_CHECK(hipStreamWaitEvent(s->stream, ready, 0));
     WL_HIP_CHECK(hipEventDestroy(ready));
+    for (size_t j = 0; j < accs.size(); ++j)
+    {
+        const wl_accel_s *a = accs[j];
+        WL_HIP_CHECK(wl::pt_arena_copy_nodes(s->d_arena_nodes, a->d_nodes, a->num_nodes,
+                                             acc_node0[j], acc_tri0[j], s->stream));
+        if (a->num_tri_records)
+            WL_HIP_CHECK(hipMemcpyAsync(s->d_arena_tris + acc_tri0[j], a->d_tris,
+                                        sizeof(WlTriRecord) * (size_t)a->num_tri_records,
+                                        hipMemcpyDeviceToDevice, s->stream));
+    }
 
     if (cfg.use_graph)
     {
         // the graph bakes in buffer pointers, grids and the bounce count
-        const uint64_t key = ((uint64_t)max_depth << 48) ^ ((uint64_t)L.stack16 << 47) ^
-                             ((uint64_t)cfg.shade_blocks_per_cu << 40) ^ (uint64_t)s->capacity ^
-                             ((uint64_t)(uintptr_t)s->d_reqs << 1) ^
-                             ((uint64_t)(uintptr_t)s->d_items << 3) ^
-                             ((uint64_t)(uintptr_t)s->d_batches << 5) | 1ull;
+        const std::vector<uint64_t> key = {
+            max_depth,
+            L.stack16,
+            cfg.shade_blocks_per_cu,
+            s->capacity,
+            (uint64_t)(uintptr_t)s->d_buf,
+            (uint64_t)(uintptr_t)s->d_reqs,
+            (uint64_t)(uintptr_t)s->d_items,
+            (uint64_t)(uintptr_t)s->d_batches,
+            (uint64_t)(uintptr_t)s->d_arena_nodes,
+            (uint64_t)(uintptr_t)s->d_arena_tris,
+        };
         if (!s->exec || s->graph_key != key)
         {
             drop_graph(s);
@@ -402,6 +458,8 @@ wl_status wl_server_destroy(wl_server s)
     hipFree(s->d_batches);
     hipFree(s->d_batch_index);
     hipFree(s->d_header);
+    hipFree(s->d_arena_nodes);
+    hipFree(s->d_arena_tris);
     if (s->stream)
         hipStreamDestroy(s->stream);
     wl_context ctx = s->ctx;
