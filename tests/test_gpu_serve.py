@@ -74,6 +74,25 @@ def test_serve_graph_replay_twice(gpu_ctx):
     server.close()
 
 
+def test_serve_arena_refilled_per_call(gpu_ctx):
+    # the BVH arena is rebuilt on every call: a second call with another set
+    # and order of scenes (other roots at the same arena addresses, replayed
+    # graph) must still match the standalone renders bit for bit
+    import wayland_amd as wl
+    print("case: serve, second call with other scenes in the arena", flush=True)
+    tracers = _tracers(gpu_ctx, 32)
+    reqs = _requests(tracers, 17)
+    server = wl.Server(gpu_ctx, batch_paths=1 << 12)
+    server.render(reqs)
+    second = [r for r in reqs if r["pt"] is not tracers[0][0]][::-1]
+    films, _, _ = server.render(second)
+    for k, r in enumerate(second):
+        ref = _standalone(r)
+        assert np.array_equal(films[k].cpu().numpy().view(np.uint32), ref.view(np.uint32)), \
+            f"request {k}"
+    server.close()
+
+
 def test_serve_autotune_then_render(gpu_ctx):
     import wayland_amd as wl
     print("case: serve autotune", flush=True)
