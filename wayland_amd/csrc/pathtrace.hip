@@ -301,33 +301,49 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t tiles = (count + 15) / 16;
     const uint32_t nwaves = gridDim.x * (kPtBlock / 64);
-    for (uint32_t tile = blockIdx.x * (kPtBlock / 64) + threadIdx.x / 64; tile < tiles;
-         tile += nwaves)
-    {
+    // software pipeline: the hit and ray of the wave's next tile are loaded
+    // while the current tile runs through the network
+    auto load_in = [&](uint32_t tile, float4 &h, float4 &a, float4 &b) {
         const uint32_t i = tile * 16 + (lane & 15);
-        bool valid = i < count;
+        h = make_float4(0.f, __uint_as_float(WL_INVALID), 0.f, 0.f);
+        if (tile < tiles && i < count)
+        {
+            h = reinterpret_cast<const float4 *>(B.hits)[i];
+            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+            a = rp[0];
+            b = rp[1];
+        }
+    };
+    const uint32_t first = blockIdx.x * (kPtBlock / 64) + threadIdx.x / 64;
+    float4 nhit, nr0 = make_float4(0.f, 0.f, 0.f, 0.f), nr1 = nr0;
+    load_in(first, nhit, nr0, nr1);
+    for (uint32_t tile = first; tile < tiles; tile += nwaves)
+    {
+        const float4 hit = nhit, r0 = nr0, r1 = nr1;
+        load_in(tile + nwaves, nhit, nr0, nr1);
+        const uint32_t i = tile * 16 + (lane & 15);
+        const uint32_t prim = __float_as_uint(hit.y);
+        bool valid = i < count && prim != WL_INVALID;
+        float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0, w2 = w0;
+        if (valid)
+        {
+            const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
+            w0 = sr[0];
+            w1 = sr[1];
+            w2 = sr[2];
+        }
 #pragma unroll
         for (int k = 0; k < 9; ++k)
             q[k] = 0.f;
         if (valid)
         {
-            const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
-            const uint32_t prim = __float_as_uint(hit.y);
-            valid = prim != WL_INVALID;
-            if (valid)
-            {
-                const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-                const float4 r0 = rp[0], r1 = rp[1];
-                const V3 o = mk(r0.x, r0.y, r0.z), d = mk(r1.x, r1.y, r1.z);
-                const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
-                const float4 w0 = sr[0], w1 = sr[1], w2 = sr[2];
-                const V3 va = mk(w0.x, w0.y, w0.z), vb = mk(w0.w, w1.x, w1.y),
-                         vc = mk(w1.z, w1.w, w2.x);
-                V3 Ng = normalize(cross(vb - va, vc - va));
-                if (dot(Ng, d) > 0)
-                    Ng = -Ng;
-                guide_query(P, o + d * hit.x, Ng, d, q);
-            }
+            const V3 o = mk(r0.x, r0.y, r0.z), d = mk(r1.x, r1.y, r1.z);
+            const V3 va = mk(w0.x, w0.y, w0.z), vb = mk(w0.w, w1.x, w1.y),
+                     vc = mk(w1.z, w1.w, w2.x);
+            V3 Ng = normalize(cross(vb - va, vc - va));
+            if (dot(Ng, d) > 0)
+                Ng = -Ng;
+            guide_query(P, o + d * hit.x, Ng, d, q);
         }
         bf16x8 x[2];
         guide_encode(q, g, x);
