@@ -733,8 +733,8 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
             },
             add);
     else
-        trav_persistent<true, kPtBlock, false>(S.nodes, S.tris, count, stack + threadIdx.x,
-                                               load, add);
+        trav_persistent<true, kPtBlock, false, false, false, 2>(
+            S.nodes, S.tris, count, stack + threadIdx.x, load, add);
 }
 
 // ---- batched serving: raygen and film update of a multi-request megabatch ----

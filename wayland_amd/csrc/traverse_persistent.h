@@ -65,8 +65,9 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // 64 rays), so waves whose share is expensive (a stretch of a multi-scene
 // queue that hits the largest BVH) get cheaper chunks too.  It suits queues
 // whose cost varies along the queue (measured best: 4 for 2^23 incoherent
-// rays, 8 for multi-request queues); the single-scene path tracer keeps
-// contiguous ranges, which keep its coherent queues together.
+// rays, 8 for multi-request queues, 2 for the path tracer's shadow rays); the
+// path tracer's extension rays keep contiguous ranges, which keep its
+// coherent queues together.
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
           uint32_t CHUNKS = 0, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
