@@ -65,6 +65,14 @@ struct wl_scene_s
     void *d_batch = nullptr;                  // one allocation, carved up
     uint32_t *d_counters = nullptr;
     unsigned long long *d_stats = nullptr;
+    // second batch lane: a large unguided wl_pt_render splits each batch into
+    // two pixel halves that run concurrently, the second on stream2
+    uint32_t capacity2 = 0;
+    void *d_batch2 = nullptr;
+    uint32_t *d_counters2 = nullptr;
+    unsigned long long *d_stats2 = nullptr;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // learned guiding
     wl_guide guide = nullptr;
     uint32_t guide_mode = 0;

@@ -172,6 +172,34 @@ wl_status ensure_capacity(wl_scene s, uint32_t paths)
     return WL_SUCCESS;
 }
 
+// The second batch lane (buffers, counters, stream, fork/join events),
+// created on first use and grown like the first.
+wl_status ensure_lane2(wl_scene s, uint32_t paths)
+{
+    if (!s->stream2)
+        WL_HIP_CHECK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
+    if (!s->ev_fork)
+        WL_HIP_CHECK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    if (!s->ev_join)
+        WL_HIP_CHECK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+    if (!s->d_counters2)
+        WL_HIP_CHECK(hipMalloc((void **)&s->d_counters2, 64));
+    if (!s->d_stats2)
+        WL_HIP_CHECK(hipMalloc((void **)&s->d_stats2, 64));
+    if (paths <= s->capacity2)
+        return WL_SUCCESS;
+    if (s->d_batch2)
+    {
+        WL_HIP_CHECK(hipDeviceSynchronize());
+        WL_HIP_CHECK(hipFree(s->d_batch2));
+        s->d_batch2 = nullptr;
+        s->capacity2 = 0;
+    }
+    WL_HIP_CHECK(hipMalloc(&s->d_batch2, batch_bytes(paths)));
+    s->capacity2 = paths;
+    return WL_SUCCESS;
+}
+
 // Guided batches: after the bounces, move the training records into the
 // guide's buffer, train, and reset the sample count.
 wl_status after_batch(wl_scene s, const wl::PtParams &P, const wl::PtBuffers &B, hipStream_t hs)
@@ -213,6 +241,8 @@ wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
 // Paths per batch when the caller gives none: 2^26 paths hold 16 GB of path
 // state and queues, so the later, shorter bounces still fill the GPU.
 constexpr uint32_t kDefaultBatchPaths = 1u << 26;
+// batches of at least this many paths run as two concurrent halves
+constexpr uint64_t kSplitMinPaths = 1u << 22;
 
 } // namespace
 
@@ -367,6 +397,15 @@ wl_status wl_scene_destroy(wl_scene s)
     hipFree(s->d_batch);
     hipFree(s->d_counters);
     hipFree(s->d_stats);
+    hipFree(s->d_batch2);
+    hipFree(s->d_counters2);
+    hipFree(s->d_stats2);
+    if (s->ev_fork)
+        hipEventDestroy(s->ev_fork);
+    if (s->ev_join)
+        hipEventDestroy(s->ev_join);
+    if (s->stream2)
+        hipStreamDestroy(s->stream2);
     hipFree(s->d_gbatch);
     hipFree(s->d_uvs);
     hipFree(s->d_textures);
@@ -494,6 +533,73 @@ wl_status wl_scene_set_camera(wl_scene s, const wl_pinhole_camera *camera)
     return WL_SUCCESS;
 }
 
+// Splits each batch of wl_pt_render into two pixel halves, [p0, p0 + h)
+// on the caller's stream and [p0 + h, p0 + np) on stream2.
+static wl_status render_two_lanes(wl_scene s, const wl_pt_config *c, float *film,
+                                  wl_pt_stats *stats, hipStream_t hs, uint32_t tile, uint32_t ns,
+                                  std::chrono::steady_clock::time_point t0)
+{
+    const uint32_t npix_total = c->width * c->height;
+    const uint32_t half = (tile + 1) / 2;
+    wl_status st = ensure_capacity(s, half * ns);
+    if (st == WL_SUCCESS)
+        st = ensure_lane2(s, half * ns);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtBuffers B = carve(s);
+    wl::PtBuffers B2 = wl::pt_carve(s->d_batch2, s->capacity2, s->d_counters2, s->d_stats2);
+    wl::PtScene S = scene_view(s);
+    hipStream_t h2 = s->stream2;
+    // fork: the second lane starts after the caller's earlier work
+    WL_HIP_CHECK(hipEventRecord(s->ev_fork, hs));
+    WL_HIP_CHECK(hipStreamWaitEvent(h2, s->ev_fork, 0));
+    if (stats)
+    {
+        WL_HIP_CHECK(hipMemsetAsync(s->d_stats, 0, 16, hs));
+        WL_HIP_CHECK(hipMemsetAsync(s->d_stats2, 0, 16, h2));
+    }
+    for (uint32_t s0 = 0; s0 < c->spp; s0 += ns)
+    {
+        const uint32_t nss = std::min(ns, c->spp - s0);
+        for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
+        {
+            const uint32_t np = std::min(tile, npix_total - p0);
+            const uint32_t na = (np + 1) / 2, nb = np - na;
+            wl::PtParams P = params_for(s, c);
+            P.nsamples = nss;
+            P.frame0 = c->frame_offset + s0;
+            P.pix0 = p0;
+            P.npix = na;
+            WL_HIP_CHECK(wl::pt_run_batch(P, S, B, s->ctx->num_cus, hs));
+            WL_HIP_CHECK(wl::pt_accumulate(P, B, film, s->ctx->num_cus, hs));
+            if (nb)
+            {
+                P.pix0 = p0 + na;
+                P.npix = nb;
+                WL_HIP_CHECK(wl::pt_run_batch(P, S, B2, s->ctx->num_cus, h2));
+                WL_HIP_CHECK(wl::pt_accumulate(P, B2, film, s->ctx->num_cus, h2));
+            }
+        }
+    }
+    // join: the caller's stream continues after both lanes
+    WL_HIP_CHECK(hipEventRecord(s->ev_join, h2));
+    WL_HIP_CHECK(hipStreamWaitEvent(hs, s->ev_join, 0));
+    if (stats)
+    {
+        unsigned long long h[2] = { 0, 0 }, h2s[2] = { 0, 0 };
+        WL_HIP_CHECK(hipMemcpyAsync(h, s->d_stats, 16, hipMemcpyDeviceToHost, hs));
+        WL_HIP_CHECK(hipMemcpyAsync(h2s, s->d_stats2, 16, hipMemcpyDeviceToHost, hs));
+        WL_HIP_CHECK(hipStreamSynchronize(hs));
+        auto t1 = std::chrono::steady_clock::now();
+        stats->paths = (uint64_t)npix_total * c->spp;
+        stats->extension_rays = h[0] + h2s[0];
+        stats->shadow_rays = h[1] + h2s[1];
+        stats->max_bounce = c->max_depth;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return WL_SUCCESS;
+}
+
 wl_status wl_pt_render(wl_scene s, const wl_pt_config *c, float *film,
                        wl_pt_stats *stats, void *stream)
 {
@@ -512,6 +618,13 @@ wl_status wl_pt_render(wl_scene s, const wl_pt_config *c, float *film,
     // batch = a pixel range x a sample range
     uint32_t tile = std::min(npix_total, max_paths);
     uint32_t ns = std::max<uint32_t>(1, std::min<uint32_t>(c->spp, max_paths / tile));
+    // Large unguided batches run as two concurrent pixel halves (the second
+    // on the scene's second stream, with its own buffers): each half's
+    // kernels fill the other's tails.  A pixel's samples stay in one half and
+    // in order, so the film is the same as from one lane.
+    const bool guided = s->guide && s->guide_mode;
+    if (!guided && (uint64_t)tile * ns >= kSplitMinPaths && tile >= 2)
+        return render_two_lanes(s, c, film, stats, hs, tile, ns, t0);
     st = ensure_capacity(s, tile * ns);
     if (st != WL_SUCCESS)
         return st;
