@@ -380,7 +380,9 @@ typedef struct wl_serve_config
                                       the largest request's pixel count     */
     uint32_t shade_blocks_per_cu;  /* shade kernel workgroups per CU        */
     uint32_t use_graph;            /* replay a captured hipGraph            */
-    uint32_t pad_;
+    uint32_t lanes;                /* 2: requests split into two groups whose
+                                      megabatches run concurrently (own
+                                      buffers and stream each); 1: one      */
 } wl_serve_config;
 
 typedef struct wl_serve_stats
@@ -401,8 +403,9 @@ wl_status wl_server_set_config(wl_server server, const wl_serve_config *cfg);
  * completion time from the start of the first megabatch. */
 wl_status wl_server_render(wl_server server, const wl_render_request *reqs, uint32_t n,
                            wl_serve_stats *stats, double *done_seconds, void *stream);
-/* Times candidate megabatch sizes and shade grids on the requests (each cut
- * to max_spp samples, rendered into scratch films), keeps the fastest. */
+/* Times candidate megabatch sizes, shade grids and lane counts on the
+ * requests (each cut to max_spp samples, rendered into scratch films), keeps
+ * the fastest. */
 wl_status wl_server_autotune(wl_server server, const wl_render_request *reqs, uint32_t n,
                              uint32_t max_spp, wl_serve_config *chosen);
 

@@ -40,13 +40,16 @@ def _standalone(r):
     return film.cpu().numpy()
 
 
-@pytest.mark.parametrize("use_graph", [1, 0])
-def test_serve_matches_standalone_bitwise(gpu_ctx, use_graph):
+@pytest.mark.parametrize("use_graph,lanes", [(1, 2), (0, 2), (1, 1)])
+def test_serve_matches_standalone_bitwise(gpu_ctx, use_graph, lanes):
+    # lanes = 2: the requests are split into two groups whose megabatches run
+    # concurrently on two streams
     import wayland_amd as wl
-    print(f"case: serve 9 requests, batch 2^13 paths, graph {use_graph}", flush=True)
+    print(f"case: serve 9 requests, batch 2^13 paths, graph {use_graph}, lanes {lanes}",
+          flush=True)
     tracers = _tracers(gpu_ctx, 64)
     reqs = _requests(tracers, 7)
-    server = wl.Server(gpu_ctx, batch_paths=1 << 13, use_graph=use_graph)
+    server = wl.Server(gpu_ctx, batch_paths=1 << 13, use_graph=use_graph, lanes=lanes)
     films, stats, done = server.render(reqs)
     print(f"batches {stats['batches']}, paths {stats['paths']}, seconds {stats['seconds']:.4f}")
     assert stats["paths"] == sum(r["width"] * r["height"] * r["spp"] for r in reqs)
@@ -103,6 +106,7 @@ def test_serve_autotune_then_render(gpu_ctx):
     print(f"chosen {cfg}")
     assert cfg["batch_paths"] in (1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26)
     assert cfg["shade_blocks_per_cu"] in (4, 8, 16)
+    assert cfg["lanes"] in (1, 2)
     films, _, _ = server.render(reqs[:3])
     for k in range(3):
         ref = _standalone(reqs[k])

@@ -433,7 +433,7 @@ class Server:
         c = _lib.ServeConfig()
         _lib.call("wl_server_get_config", self.h, C.byref(c))
         return {"batch_paths": c.batch_paths, "shade_blocks_per_cu": c.shade_blocks_per_cu,
-                "use_graph": c.use_graph}
+                "use_graph": c.use_graph, "lanes": c.lanes}
 
     def set_config(self, **cfg):
         c = _lib.ServeConfig()
@@ -457,7 +457,7 @@ class Server:
         c = _lib.ServeConfig()
         _lib.call("wl_server_autotune", self.h, arr, len(reqs), int(max_spp), C.byref(c))
         return {"batch_paths": c.batch_paths, "shade_blocks_per_cu": c.shade_blocks_per_cu,
-                "use_graph": c.use_graph}
+                "use_graph": c.use_graph, "lanes": c.lanes}
 
     def close(self):
         if getattr(self, "h", None):

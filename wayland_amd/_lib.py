@@ -96,7 +96,7 @@ class RenderRequest(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("batch_paths", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
-                ("use_graph", C.c_uint32), ("pad_", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("lanes", C.c_uint32)]
 
 
 class ServeStats(C.Structure):

@@ -7,8 +7,11 @@
 // extend / shade / shadow, film update) are captured once into a hipGraph,
 // and the graph is replayed once per megabatch: the batch descriptors live in
 // device memory and the graph's first kernel picks the next one, so the host
-// enqueues nothing but graph launches.  An autotuner times candidate
-// megabatch sizes and shade grids on the workload itself.
+// enqueues nothing but graph launches.  With two lanes the requests are
+// split into two groups (each request whole in one group) whose megabatch
+// sequences run concurrently on two streams with their own buffers, so each
+// lane's kernels fill the other's tails.  An autotuner times candidate
+// megabatch sizes, shade grids and lane counts on the workload itself.
 //
 // Each request's film equals its standalone wl_pt_render result bit for bit:
 // a path's arithmetic depends only on its own state and its request's scene,
@@ -52,6 +55,10 @@ struct wl_server_s
     hipGraphExec_t exec = nullptr;
     std::vector<uint64_t> graph_key;
     std::vector<hipEvent_t> events;
+    // second lane (created on first use): with cfg.lanes = 2 the requests are
+    // split into two groups whose megabatches run concurrently, the second
+    // group through this server's twin (own buffers, stream and graph)
+    wl_server_s *twin = nullptr;
 };
 
 namespace
@@ -129,6 +136,10 @@ wl_status normalise_config(wl_serve_config &c)
         return set_error(WL_ERROR_LIMIT_EXCEEDED, "batch_paths above 2^26");
     if (c.shade_blocks_per_cu > 64)
         return set_error(WL_ERROR_INVALID_VALUE, "shade_blocks_per_cu above 64");
+    if (c.lanes == 0)
+        c.lanes = 1;
+    if (c.lanes > 2)
+        return set_error(WL_ERROR_INVALID_VALUE, "lanes above 2");
     return WL_SUCCESS;
 }
 
@@ -189,18 +200,20 @@ void plan(const wl_render_request *r, uint32_t n, uint32_t capacity, Plan &P)
     }
 }
 
-wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_stats *stats,
-              double *done_seconds, hipStream_t user)
+// What a lane enqueued, for finish_lane.
+struct LaneRun
 {
-    if (!s || (n && !reqs))
-        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
-    if (n == 0)
-        return WL_SUCCESS;
-    wl_serve_config cfg = s->cfg;
-    wl_status st = normalise_config(cfg);
-    if (st != WL_SUCCESS)
-        return st;
-    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    Plan pl;
+    uint32_t nb = 0;
+};
+
+// Validates the requests, builds the lane's plan and arena and enqueues its
+// megabatches on s->stream after the work already on `user`; returns without
+// waiting.
+wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_request *reqs,
+                   uint32_t n, hipStream_t user, LaneRun &out)
+{
+    wl_status st = WL_SUCCESS;
     std::vector<wl::PtReq> hreq(n);
     uint32_t max_npix = 0, max_depth = 0, max_bvh = 0;
     for (uint32_t k = 0; k < n; ++k)
@@ -252,9 +265,10 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
     if (max_npix > cfg.batch_paths)
         return set_error(WL_ERROR_LIMIT_EXCEEDED,
                          "batch_paths must hold the largest request's pixel count");
-    Plan pl;
+    Plan &pl = out.pl;
     plan(reqs, n, cfg.batch_paths, pl);
     const uint32_t nb = (uint32_t)pl.batches.size();
+    out.nb = nb;
     st = ensure_buffers(s, cfg.batch_paths);
     if (st == WL_SUCCESS)
         st = grow(&s->d_reqs, s->reqs_cap, n);
@@ -363,60 +377,129 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
             WL_HIP_CHECK(wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream));
         WL_HIP_CHECK(hipEventRecord(s->events[b + 1], s->stream));
     }
+    return WL_SUCCESS;
+}
+
+// Waits for a lane; its megabatch end times relative to `origin` (an event
+// recorded before any lane started), per request in `done` (lane order), and
+// its ray counts added to `rays`.
+wl_status finish_lane(wl_server s, const LaneRun &lr, const wl_render_request *reqs, uint32_t n,
+                      hipEvent_t origin, double *done, double &end,
+                      unsigned long long rays[2])
+{
     WL_HIP_CHECK(hipStreamSynchronize(s->stream));
-    std::vector<double> tb(nb + 1, 0.0);
-    for (uint32_t b = 1; b <= nb; ++b)
+    std::vector<double> tb(lr.nb + 1, 0.0);
+    for (uint32_t b = 0; b <= lr.nb; ++b)
     {
         float ms = 0.f;
-        WL_HIP_CHECK(hipEventElapsedTime(&ms, s->events[0], s->events[b]));
+        WL_HIP_CHECK(hipEventElapsedTime(&ms, origin, s->events[b]));
         tb[b] = ms * 1e-3;
     }
+    end = tb[lr.nb];
+    for (uint32_t k = 0; k < n; ++k)
+        done[k] = reqs[k].spp ? tb[lr.pl.last_batch[k] + 1] : 0.0;
+    unsigned long long h[2] = { 0, 0 };
+    WL_HIP_CHECK(hipMemcpy(h, s->d_stats, 16, hipMemcpyDeviceToHost));
+    rays[0] += h[0];
+    rays[1] += h[1];
+    return WL_SUCCESS;
+}
+
+// Creates the twin lane of `s` (same device and configuration); servers are
+// created and freed by create_server / destroy_server.
+wl_status make_twin(wl_server s);
+wl_status create_server(wl_context ctx, const wl_serve_config &c, wl_server *out);
+wl_status destroy_server(wl_server s);
+
+wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_stats *stats,
+              double *done_seconds, hipStream_t user)
+{
+    if (!s || (n && !reqs))
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (n == 0)
+        return WL_SUCCESS;
+    wl_serve_config cfg = s->cfg;
+    wl_status st = normalise_config(cfg);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    // requests -> lanes: each request to the lane with fewer paths so far
+    // (a request stays in one lane, so its film gets its samples in order)
+    std::vector<uint32_t> idx[2];
+    uint64_t load[2] = { 0, 0 };
+    const uint32_t lanes = (cfg.lanes == 2 && n >= 2) ? 2u : 1u;
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const uint32_t l = (lanes == 2 && load[1] < load[0]) ? 1u : 0u;
+        idx[l].push_back(k);
+        load[l] += (uint64_t)reqs[k].width * reqs[k].height * reqs[k].spp;
+    }
+    const uint32_t used = idx[1].empty() ? 1u : 2u;
+    if (used == 2 && !s->twin)
+    {
+        st = make_twin(s);
+        if (st != WL_SUCCESS)
+            return st;
+    }
+    std::vector<wl_render_request> sub[2];
+    for (uint32_t l = 0; l < used; ++l)
+        for (uint32_t k : idx[l])
+            sub[l].push_back(reqs[k]);
+    wl_server lane[2] = { s, s->twin };
+    hipEvent_t origin;
+    WL_HIP_CHECK(hipEventCreate(&origin));
+    LaneRun lr[2];
+    st = WL_SUCCESS;
+    for (uint32_t l = 0; l < used && st == WL_SUCCESS; ++l)
+    {
+        if (l == 0)
+            WL_HIP_CHECK(hipEventRecord(origin, user));
+        st = run_lane(lane[l], cfg, sub[l].data(), (uint32_t)sub[l].size(), user, lr[l]);
+    }
+    std::vector<double> done[2];
+    double end = 0.0;
+    unsigned long long rays[2] = { 0, 0 };
+    for (uint32_t l = 0; l < used && st == WL_SUCCESS; ++l)
+    {
+        done[l].assign(sub[l].size(), 0.0);
+        double e = 0.0;
+        st = finish_lane(lane[l], lr[l], sub[l].data(), (uint32_t)sub[l].size(), origin,
+                         done[l].data(), e, rays);
+        end = std::max(end, e);
+    }
+    if (st != WL_SUCCESS)
+    {
+        // a lane may still hold enqueued work: let it drain before returning
+        for (uint32_t l = 0; l < used; ++l)
+            if (lane[l])
+                hipStreamSynchronize(lane[l]->stream);
+    }
+    hipEventDestroy(origin);
+    if (st != WL_SUCCESS)
+        return st;
     if (done_seconds)
-        for (uint32_t k = 0; k < n; ++k)
-            done_seconds[k] = reqs[k].spp ? tb[pl.last_batch[k] + 1] : 0.0;
+        for (uint32_t l = 0; l < used; ++l)
+            for (size_t j = 0; j < idx[l].size(); ++j)
+                done_seconds[idx[l][j]] = done[l][j];
     if (stats)
     {
-        unsigned long long h[2] = { 0, 0 };
-        WL_HIP_CHECK(hipMemcpy(h, s->d_stats, 16, hipMemcpyDeviceToHost));
         std::memset(stats, 0, sizeof *stats);
-        stats->seconds = tb[nb];
-        stats->paths = pl.paths;
-        stats->extension_rays = h[0];
-        stats->shadow_rays = h[1];
-        stats->batches = nb;
+        // from the call's start (work already queued on `user` included)
+        // to the end of the last lane
+        stats->seconds = end;
+        for (uint32_t l = 0; l < used; ++l)
+        {
+            stats->paths += lr[l].pl.paths;
+            stats->batches += lr[l].nb;
+        }
+        stats->extension_rays = rays[0];
+        stats->shadow_rays = rays[1];
     }
     return WL_SUCCESS;
 }
 
-} // namespace
-
-extern "C" {
-
-wl_status wl_serve_default_config(wl_serve_config *cfg)
+wl_status create_server(wl_context ctx, const wl_serve_config &c, wl_server *out)
 {
-    if (!cfg)
-        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
-    std::memset(cfg, 0, sizeof *cfg);
-    cfg->batch_paths = kDefaultServeBatch;
-    cfg->shade_blocks_per_cu = 8;
-    cfg->use_graph = 1;
-    return WL_SUCCESS;
-}
-
-wl_status wl_server_create(wl_context ctx, const wl_serve_config *cfg, wl_server *out)
-{
-    if (!ctx || !out)
-        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
-    *out = nullptr;
-    wl_serve_config c;
-    if (cfg)
-        c = *cfg;
-    else
-        wl_serve_default_config(&c);
-    wl_status st = normalise_config(c);
-    if (st != WL_SUCCESS)
-        return st;
-    WL_HIP_CHECK(hipSetDevice(ctx->device));
     wl_server s = new (std::nothrow) wl_server_s;
     if (!s)
         return set_error(WL_ERROR_OUT_OF_MEMORY, "server allocation");
@@ -434,17 +517,22 @@ wl_status wl_server_create(wl_context ctx, const wl_serve_config *cfg, wl_server
         e = hipMalloc((void **)&s->d_header, 64);
     if (e != hipSuccess)
     {
-        wl_server_destroy(s);
+        destroy_server(s);
         return wl::hip_error(e, "server set-up");
     }
     *out = s;
     return WL_SUCCESS;
 }
 
-wl_status wl_server_destroy(wl_server s)
+wl_status make_twin(wl_server s)
 {
-    if (!s)
-        return set_error(WL_ERROR_INVALID_VALUE, "server is NULL");
+    wl_serve_config c = s->cfg;
+    c.lanes = 1;
+    return create_server(s->ctx, c, &s->twin);
+}
+
+wl_status destroy_server(wl_server s)
+{
     hipSetDevice(s->ctx->device);
     hipDeviceSynchronize();
     drop_graph(s);
@@ -466,6 +554,49 @@ wl_status wl_server_destroy(wl_server s)
     delete s;
     wl::ctx_release(ctx);
     return WL_SUCCESS;
+}
+
+} // namespace
+
+extern "C" {
+
+wl_status wl_serve_default_config(wl_serve_config *cfg)
+{
+    if (!cfg)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->batch_paths = kDefaultServeBatch;
+    cfg->shade_blocks_per_cu = 8;
+    cfg->use_graph = 1;
+    cfg->lanes = 2;
+    return WL_SUCCESS;
+}
+
+wl_status wl_server_create(wl_context ctx, const wl_serve_config *cfg, wl_server *out)
+{
+    if (!ctx || !out)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    *out = nullptr;
+    wl_serve_config c;
+    if (cfg)
+        c = *cfg;
+    else
+        wl_serve_default_config(&c);
+    wl_status st = normalise_config(c);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(hipSetDevice(ctx->device));
+    return create_server(ctx, c, out);
+}
+
+wl_status wl_server_destroy(wl_server s)
+{
+    if (!s)
+        return set_error(WL_ERROR_INVALID_VALUE, "server is NULL");
+    if (s->twin)
+        wl_server_destroy(s->twin);
+    s->twin = nullptr;
+    return destroy_server(s);
 }
 
 wl_status wl_server_get_config(wl_server s, wl_serve_config *cfg)
@@ -529,11 +660,12 @@ wl_status wl_server_autotune(wl_server s, const wl_render_request *reqs, uint32_
     {
         if (bp < max_npix)
             continue;
-        for (uint32_t sh : shades)
+        for (uint32_t cand = 0; cand < 6; ++cand)
         {
             wl_serve_config c = saved;
             c.batch_paths = bp;
-            c.shade_blocks_per_cu = sh;
+            c.shade_blocks_per_cu = shades[cand % 3];
+            c.lanes = 1 + cand / 3;
             c.use_graph = 1;
             s->cfg = c;
             wl_serve_stats a{}, b{};
