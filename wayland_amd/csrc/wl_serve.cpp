@@ -207,9 +207,34 @@ struct LaneRun
     uint32_t nb = 0;
 };
 
-// Validates the requests, builds the lane's plan and arena and enqueues its
-// megabatches on s->stream after the work already on `user`; returns without
-// waiting.
+// Checks every request before any lane enqueues work, so an invalid request
+// leaves every film untouched.
+wl_status validate_requests(wl_server s, const wl_serve_config &cfg,
+                            const wl_render_request *reqs, uint32_t n)
+{
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const wl_render_request &r = reqs[k];
+        if (!r.scene || !r.film)
+            return set_error(WL_ERROR_INVALID_VALUE, "request without scene or film");
+        if (r.scene->ctx->device != s->ctx->device)
+            return set_error(WL_ERROR_INVALID_VALUE, "request scene on another device");
+        if (r.scene->guide && r.scene->guide_mode)
+            return set_error(WL_ERROR_NOT_SUPPORTED, "batched serving of a guided scene");
+        const wl_pt_config c = request_config(r);
+        const wl_status st = wl::pt_check_config(r.scene, &c);
+        if (st != WL_SUCCESS)
+            return st;
+        if (r.width * r.height > cfg.batch_paths)
+            return set_error(WL_ERROR_LIMIT_EXCEEDED,
+                             "batch_paths must hold the largest request's pixel count");
+    }
+    return WL_SUCCESS;
+}
+
+// Builds the lane's plan and arena and enqueues its megabatches on s->stream
+// after the work already on `user` (the requests are validated); returns
+// without waiting.
 wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_request *reqs,
                    uint32_t n, hipStream_t user, LaneRun &out)
 {
@@ -423,6 +448,9 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
     if (st != WL_SUCCESS)
         return st;
     WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    st = validate_requests(s, cfg, reqs, n);
+    if (st != WL_SUCCESS)
+        return st;
     // requests -> lanes: each request to the lane with fewer paths so far
     // (a request stays in one lane, so its film gets its samples in order)
     std::vector<uint32_t> idx[2];
