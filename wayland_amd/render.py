@@ -172,7 +172,12 @@ class Renderer:
                                 rr_depth=rr_depth)
         npix = self.width * self.height
         self.radiance = torch.zeros((npix, 4), dtype=torch.float32, device="cuda")
-        self.color = torch.zeros((npix, 4), dtype=torch.uint8, device="cuda")
+        # two colour buffers and two pinned host buffers: frame k's download
+        # (on its own stream) overlaps the render of frame k + 1
+        self.color = [torch.zeros((npix, 4), dtype=torch.uint8, device="cuda") for _ in range(2)]
+        self.host = [torch.empty((npix, 4), dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+        self.copy_stream = torch.cuda.Stream()
+        self.copied = [None, None]  # event: the download from colour buffer k is done
         self.frame_buffer = np.zeros((npix, 4), np.uint8)
         self.frame_id = 0
 
@@ -181,12 +186,29 @@ class Renderer:
         s = self.setting
         if s.frame_limit <= 0 and s.time_limit <= 0:
             raise ValueError("a headless run needs a frame limit (-fr) or a time limit (-t)")
+        import torch
         self.frame_id = 0
         t0 = time.perf_counter_ns()
         frames = 0
+        main = torch.cuda.current_stream()
+        last = None  # (slot, event) of the newest download
         while True:
-            self.pt.launch_frame(self.cfg, self.frame_id, self.radiance, self.color)
-            self.frame_buffer = self.color.cpu().numpy()
+            slot = self.frame_id & 1
+            if self.copied[slot] is not None:  # that buffer's previous download
+                main.wait_event(self.copied[slot])
+            self.pt.launch_frame(self.cfg, self.frame_id, self.radiance, self.color[slot])
+            rendered = torch.cuda.Event()
+            rendered.record(main)
+            self.copy_stream.wait_event(rendered)
+            with torch.cuda.stream(self.copy_stream):
+                self.host[slot].copy_(self.color[slot], non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(self.copy_stream)
+            self.copied[slot] = done
+            if last is not None:  # the previous frame is on the host by now
+                last[1].synchronize()
+                self.frame_buffer = self.host[last[0]].numpy()
+            last = (slot, done)
             self.frame_id += 1
             frames += 1
             dur_ns = time.perf_counter_ns() - t0
@@ -194,6 +216,9 @@ class Renderer:
             out_frame = s.frame_limit > 0 and frames >= s.frame_limit
             if out_time or out_frame:
                 break
+        last[1].synchronize()  # the last frame, kept past later runs
+        self.frame_buffer = self.host[last[0]].numpy().copy()
+        dur_ns = time.perf_counter_ns() - t0  # up to the last frame on the host
         seconds = float(np.float32(dur_ns // 1000) / np.float32(1000) / np.float32(1000))
         if s.write_output:
             self.write_output(seconds, frames)
