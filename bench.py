@@ -421,9 +421,10 @@ def cpu_baseline(tris, rays_np):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are "
-                                   "not part of it (the path shards by rays/samples, "
-                                   "see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the path would "
+                                   "shard by rays (traversal) and by sample ranges "
+                                   "(path tracer), which is not implemented here",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     import wayland_amd as wl
