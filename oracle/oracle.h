@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library includes, links
  * or calls this code; only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg load liboracle.so, and only as the checker / the timed CPU
- * baseline.  It is a plain-C restatement of the reference algorithm
+ * cpu_baseline leg load liboracle.so, and only as the reference results the
+ * GPU path is compared against / the timed CPU baseline.  It is a plain-C restatement of the reference algorithm
  * (Extra-Creativity/Wayland, "EasyRender"); each function cites the reference
  * file:line it follows.  The reference itself cannot be built here (it needs
  * the CUDA toolkit, the OptiX SDK, glm and spdlog), so there is no oracle/_ref.

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests and
-the benchmark's CPU-baseline leg.  The oracle is the checker: nothing in the
-shipped package imports this module."""
+the benchmark's CPU-baseline leg.  The oracle is what the GPU results are
+compared against: nothing in the shipped package imports this module."""
 from __future__ import annotations
 
 import ctypes as C

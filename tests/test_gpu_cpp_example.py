@@ -2,8 +2,8 @@
 (tests/cpp/example_pathtracer.cpp), run in-process through its exported
 entry point on the reference's cornell.obj: the colour buffer matches the
 oracle's restatement of PathTracer.cu as the Python launch test does, and
-the bytes equal FromFloatToChar of it.  (The driver also calls the denoiser,
-which throws "not supported"; it prints that and goes on.)"""
+the bytes equal FromFloatToChar of it.  (The example's main also calls the
+denoiser, which throws "not supported"; it prints that and goes on.)"""
 import ctypes
 import os
 import struct
