@@ -6,13 +6,15 @@
 //
 //   raygen  -> [extend -> shade -> shadow] x max_depth -> accumulate
 //
-//   raygen     TEA seed (pixel, frame), pinhole camera ray, path state.
+//   raygen     TEA seed (pixel, frame), pinhole camera ray, zeroed radiance
+//              (the first shade pass recomputes the rest of the first
+//              vertex's state instead of reading it).
 //   extend     closest-hit BVH8 traversal of the active ray queue.
 //   shade      miss / emitter hit with MIS / depth cap / Disney BSDF
 //              sample-pdf-eval / NEE light sample + BSDF eval / RR, in the
 //              reference's RNG draw order; appends the continuing paths and
 //              the shadow rays to compacted queues (wave64 ballot + mbcnt
-//              prefix, one atomic per wave).
+//              prefix, one atomic per queue per workgroup).
 //   shadow     any-hit traversal of the shadow queue; adds the precomputed
 //              NEE contribution of unoccluded rays.
 //   accumulate compensated (Kahan) FP32 film: sum + compensation per pixel.
@@ -103,25 +105,34 @@ __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.
 __device__ __forceinline__ V3 v3of(float4 a) { return mk(a.x, a.y, a.z); }
 
 // ---- raygen -------------------------------------------------------------
+// Camera ray of path p of a batch: its pixel and sample, the seed after
+// PinholeGenerateRay's two draws (Camera.h:28-35: u, then v), the direction.
+__device__ __forceinline__ void camera_ray(const PtParams &P, uint32_t p, V3 &o, V3 &d,
+                                           uint32_t &seed)
+{
+    const uint32_t pixel = P.pix0 + p % P.npix;
+    const uint32_t s = p / P.npix;
+    const uint32_t x = pixel % P.width, y = pixel / P.width;
+    seed = tea4(pixel, P.frame0 + s);
+    const float u = 2.f * ((float)(int)x + rnd(seed)) / (float)(int)P.width - 1.f;
+    const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
+    d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
+    o = ld3(P.cam.pos);
+}
+
+// Single-scene batches: the camera rays and zeroed radiance.  The rest of
+// the first vertex's path state (origin, direction and seed, throughput 1,
+// depth 0, lastTraceTerm 1, lastPdf 1) is not stored: the first shade pass
+// recomputes it with camera_ray, saving 64 B of writes and reads per path.
 __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
 {
     const uint32_t npaths = P.npix * P.nsamples;
     for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < npaths;
          p += gridDim.x * kPtBlock)
     {
-        const uint32_t pixel = P.pix0 + p % P.npix;
-        const uint32_t s = p / P.npix;
-        const uint32_t x = pixel % P.width, y = pixel / P.width;
-        uint32_t seed = tea4(pixel, P.frame0 + s);
-        // PinholeGenerateRay (Camera.h:28-35): two draws, u then v
-        const float u = 2.f * ((float)(int)x + rnd(seed)) / (float)(int)P.width - 1.f;
-        const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
-        const V3 d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
-        const V3 o = ld3(P.cam.pos);
-        B.pos[p] = f4(o, 1.0f);                          // w: lastPdf
-        B.dir[p] = f4(d, __uint_as_float(seed));         // w: seed
-        B.thr[p] = f4(mk(1, 1, 1), __uint_as_float(0u)); // w: depth
-        B.ltt[p] = f4(mk(1, 1, 1), 0.0f);                // lastTraceTerm
+        V3 o, d;
+        uint32_t seed;
+        camera_ray(P, p, o, d, seed);
         B.rad[p] = f4(mk(0, 0, 0), 0.0f);
         B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
         B.qpath0[p] = p;
@@ -366,7 +377,8 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 // vertex is summed; GSAMPLE vertices record themselves).
 template <bool GSAMPLE, bool GTRAIN, bool MULTI>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3)))
-void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
+void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__restrict__ reqs,
+             int first)
 {
     const uint32_t count = B.counters[parity];
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
@@ -397,8 +409,25 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             const PtParams &P = MULTI ? rq->P : P0;
             const PtScene &S = MULTI ? rq->S : S0;
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
-            const float4 s_pos = B.pos[path], s_dir = B.dir[path], s_thr = B.thr[path];
-            const float4 s_ltt = B.ltt[path];
+            float4 s_pos, s_dir, s_thr, s_ltt;
+            if (!MULTI && first)
+            {
+                // the first vertex: the state k_raygen did not store
+                V3 o, d;
+                uint32_t sd;
+                camera_ray(P, path, o, d, sd);
+                s_pos = f4(o, 1.0f);                          // w: lastPdf
+                s_dir = f4(d, __uint_as_float(sd));           // w: seed
+                s_thr = f4(mk(1, 1, 1), __uint_as_float(0u)); // w: depth
+                s_ltt = f4(mk(1, 1, 1), 0.0f);
+            }
+            else
+            {
+                s_pos = B.pos[path];
+                s_dir = B.dir[path];
+                s_thr = B.thr[path];
+                s_ltt = B.ltt[path];
+            }
             V3 rayPos = v3of(s_pos);
             float lastPdf = s_pos.w;
             V3 rayDir = v3of(s_dir);
@@ -1014,17 +1043,17 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                                parity);
             if (gtrain)
                 hipLaunchKernelGGL((k_shade<true, true, false>), dim3(g_shade), dim3(kPtBlock), 0,
-                                   st, P, S, B, parity, nullptr);
+                                   st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
             else
                 hipLaunchKernelGGL((k_shade<true, false, false>), dim3(g_shade), dim3(kPtBlock),
-                                   0, st, P, S, B, parity, nullptr);
+                                   0, st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
         }
         else if (gtrain)
             hipLaunchKernelGGL((k_shade<false, true, false>), dim3(g_shade), dim3(kPtBlock), 0, st,
-                               P, S, B, parity, nullptr);
+                               P, S, B, parity, nullptr, b == 0 ? 1 : 0);
         else
             hipLaunchKernelGGL((k_shade<false, false, false>), dim3(g_shade), dim3(kPtBlock), 0,
-                               st, P, S, B, parity, nullptr);
+                               st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
         if (shallow)
             hipLaunchKernelGGL((k_shadow<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
                                nullptr);
@@ -1080,7 +1109,7 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
                                dim3(kPtBlock), 0, st, SA, B, parity, L.reqs);
         hipLaunchKernelGGL((k_shade<false, false, true>), dim3(g_shade), dim3(kPtBlock), 0, st,
-                           P0, S0, B, parity, L.reqs);
+                           P0, S0, B, parity, L.reqs, 0);
         if (L.stack16)
             hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, SA, B,
                                L.reqs);
