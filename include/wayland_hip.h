@@ -261,7 +261,10 @@ typedef struct wl_pt_stats
 /* Adds cfg->spp samples per pixel into a compensated FP32 film:
  * d_film = 6 floats per pixel (sum RGB, Kahan compensation RGB), caller
  * zeroes it before the first call.  Mean = sum / total samples.  If stats is
- * non-NULL the call synchronises and fills it. */
+ * non-NULL the call synchronises and fills it.  Without a guide, batches of
+ * at least 2^22 paths run as two pixel halves at once, the second on a stream
+ * of the scene; `stream` then waits on an event after both, so work enqueued
+ * on it later is ordered after the whole render. */
 wl_status wl_pt_render(wl_scene scene, const wl_pt_config *cfg, float *d_film,
                        wl_pt_stats *stats, void *stream);
 /* The per-sample radiance of frame `frame` (one sample per pixel, TEA seed
