@@ -410,14 +410,24 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             const PtScene &S = MULTI ? rq->S : S0;
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
             float4 s_pos, s_dir, s_thr, s_ltt;
-            if (!MULTI && first)
+            if (first)
             {
-                // the first vertex: the state k_raygen did not store
-                V3 o, d;
-                uint32_t sd;
-                camera_ray(P, path, o, d, sd);
-                s_pos = f4(o, 1.0f);                          // w: lastPdf
-                s_dir = f4(d, __uint_as_float(sd));           // w: seed
+                // the first vertex: the state raygen did not store (the
+                // multi-request raygen keeps the direction and seed, whose
+                // recomputation would need the item lookup)
+                if (MULTI)
+                {
+                    s_pos = f4(ld3(P.cam.pos), 1.0f);
+                    s_dir = B.dir[path];
+                }
+                else
+                {
+                    V3 o, d;
+                    uint32_t sd;
+                    camera_ray(P, path, o, d, sd);
+                    s_pos = f4(o, 1.0f);                // w: lastPdf
+                    s_dir = f4(d, __uint_as_float(sd)); // w: seed
+                }
                 s_thr = f4(mk(1, 1, 1), __uint_as_float(0u)); // w: depth
                 s_ltt = f4(mk(1, 1, 1), 0.0f);
             }
@@ -816,10 +826,9 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen_multi(PtMultiLaunch L, PtBu
         const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
         const V3 d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
         const V3 o = ld3(P.cam.pos);
-        B.pos[p] = f4(o, 1.0f);
+        // origin, throughput, depth and lastTraceTerm are recomputed by the
+        // first shade pass
         B.dir[p] = f4(d, __uint_as_float(seed));
-        B.thr[p] = f4(mk(1, 1, 1), __uint_as_float(0u));
-        B.ltt[p] = f4(mk(1, 1, 1), 0.0f);
         B.rad[p] = f4(mk(0, 0, 0), 0.0f);
         B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
         B.qpath0[p] = p;
@@ -1109,7 +1118,7 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
                                dim3(kPtBlock), 0, st, SA, B, parity, L.reqs);
         hipLaunchKernelGGL((k_shade<false, false, true>), dim3(g_shade), dim3(kPtBlock), 0, st,
-                           P0, S0, B, parity, L.reqs, 0);
+                           P0, S0, B, parity, L.reqs, b == 0 ? 1 : 0);
         if (L.stack16)
             hipLaunchKernelGGL((k_shadow<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, SA, B,
                                L.reqs);
