@@ -164,7 +164,7 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     auto load = [&](uint32_t i, TravRay &r) { load_ray_nt(rays, i, r); };
     auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
     if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
-        trav_persistent<false, kPtBlock, false, true, false, 8>(
+        trav_persistent<false, kPtBlock, false, true, false, 8, 20>(
             S.nodes, S.tris, count, stack + threadIdx.x,
             [&](uint32_t i, TravRay &r, uint32_t &root) {
                 root = reqs[qreq[i]].root;
@@ -172,8 +172,8 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
             },
             store);
     else
-        trav_persistent<false, kPtBlock, false>(S.nodes, S.tris, count, stack + threadIdx.x,
-                                                load, store);
+        trav_persistent<false, kPtBlock, false, false, false, 0, 20>(
+            S.nodes, S.tris, count, stack + threadIdx.x, load, store);
 }
 
 // ---- material sort of the shade queue ------------------------------------------
@@ -764,7 +764,7 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
         }
     };
     if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
-        trav_persistent<true, kPtBlock, false, true, false, 8>(
+        trav_persistent<true, kPtBlock, false, true, false, 8, 20>(
             S.nodes, S.tris, count, stack + threadIdx.x,
             [&](uint32_t i, TravRay &r, uint32_t &root) {
                 root = B.sreq[i];
@@ -772,7 +772,7 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
             },
             add);
     else
-        trav_persistent<true, kPtBlock, false, false, false, 2>(
+        trav_persistent<true, kPtBlock, false, false, false, 2, 20>(
             S.nodes, S.tris, count, stack + threadIdx.x, load, add);
 }
 

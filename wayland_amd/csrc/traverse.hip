@@ -28,7 +28,7 @@ void k_trace_closest_p(
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
-    trav_persistent<false, kTraceBlock, false, false, true, 4>(
+    trav_persistent<false, kTraceBlock, false, false, true, 4, 24>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); });

@@ -4,7 +4,7 @@
 // that runs one ray per lane to completion idles most of its lanes near the
 // end.  Here a fixed grid of waves loops: every lane advances its ray by one
 // step (any-hit: a node and its triangles, trav_step; closest hit: one node
-// or one triangle, trav_step_one); once at least kRefillLanes lanes of a
+// or one triangle, trav_step_one); once at least REFILL lanes of a
 // wave are idle, the idle lanes start the next rays of the wave's own ray
 // indices (after Aila & Laine, "Understanding the Efficiency of Ray Traversal
 // on GPUs", HPG 2009).
@@ -41,6 +41,10 @@ __device__ __forceinline__ void store_hit_nt(wl_hit *__restrict__ hits, uint32_t
     __builtin_nontemporal_store(v, reinterpret_cast<f4v *>(hits) + i);
 }
 
+// Default refill threshold: a wave refills once at least this many of its
+// lanes are idle (REFILL below overrides it per kernel; measured best: 24 for
+// 2^23 incoherent closest-hit rays, 20 for the path tracer's queues, 16 for
+// raw any-hit).
 constexpr int kRefillLanes = 16;
 
 __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
@@ -69,7 +73,7 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // path tracer's extension rays keep contiguous ranges, which keep its
 // coherent queues together.
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
-          uint32_t CHUNKS = 0, class Fetch, class Emit>
+          uint32_t CHUNKS = 0, int REFILL = kRefillLanes, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
                                                 const WlTriRecord *__restrict__ tris,
                                                 uint32_t n, uint32_t *stack_lane,
@@ -104,7 +108,7 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
     {
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
-        if (next < end && nidle >= (uint32_t)kRefillLanes)
+        if (next < end && nidle >= (uint32_t)REFILL)
         {
             if (!alive)
             {
