@@ -118,6 +118,31 @@ double oracle_disney_pdf_f64(const oracle_material *m, const double Ns[3],
 void oracle_disney_sample_f64(const oracle_material *m, const double Ns[3],
                               const double Ng[3], const double V[3],
                               uint32_t *seed, double L[3]);
+/* Batched forms for one (material, Ns, Ng, V): seeds -> directions (3n),
+ * directions (3n) -> pdfs (n), directions (3n) -> eval values (3n). */
+void oracle_disney_sample_batch_f32(const oracle_material *m, const float Ns[3],
+                                    const float Ng[3], const float V[3],
+                                    const uint32_t *seeds, uint32_t n, float *L);
+void oracle_disney_pdf_batch_f32(const oracle_material *m, const float Ns[3],
+                                 const float Ng[3], const float V[3], const float *L,
+                                 uint32_t n, float *pdf);
+void oracle_disney_eval_batch_f32(const oracle_material *m, const float Ns[3],
+                                  const float Ng[3], const float V[3], const float *L,
+                                  uint32_t n, float *out);
+void oracle_disney_sample_batch_f64(const oracle_material *m, const double Ns[3],
+                                    const double Ng[3], const double V[3],
+                                    const uint32_t *seeds, uint32_t n, double *L);
+void oracle_disney_pdf_batch_f64(const oracle_material *m, const double Ns[3],
+                                 const double Ng[3], const double V[3], const double *L,
+                                 uint32_t n, double *pdf);
+void oracle_disney_eval_batch_f64(const oracle_material *m, const double Ns[3],
+                                  const double Ng[3], const double V[3], const double *L,
+                                  uint32_t n, double *out);
+/* MaterialMath (Common.h:119-181), one function at a time: which = 0
+ * SchlickFresnel(u), 1 Fresnel(cos_i, cos_t, eta), 2 GTR1(NdotH, a),
+ * 3 GTR2(NdotH, a), 4 SmithG_GGX(NdotV, alphaG), 5 Lambda(w[3], n[3]). */
+float oracle_material_math_f32(int which, const float *args);
+double oracle_material_math_f64(int which, const double *args);
 /* Orthonormal basis, Common.h:102-116 (Duff et al. 2017). */
 void oracle_onb_f32(const float n[3], float u[3], float v[3]);
 /* Texture fetch of the reference's texture object (wrap, bilinear with 8-bit

@@ -91,6 +91,15 @@ def lib():
             getattr(L, f"oracle_disney_pdf_{sfx}").argtypes = [C.c_void_p, fp, fp, fp, fp]
             getattr(L, f"oracle_disney_pdf_{sfx}").restype = C.c_float if sfx == "f32" else C.c_double
             getattr(L, f"oracle_disney_sample_{sfx}").argtypes = [C.c_void_p, fp, fp, fp, u32p, fp]
+            getattr(L, f"oracle_disney_sample_batch_{sfx}").argtypes = [
+                C.c_void_p, fp, fp, fp, u32p, C.c_uint32, fp]
+            getattr(L, f"oracle_disney_pdf_batch_{sfx}").argtypes = [
+                C.c_void_p, fp, fp, fp, fp, C.c_uint32, fp]
+            getattr(L, f"oracle_disney_eval_batch_{sfx}").argtypes = [
+                C.c_void_p, fp, fp, fp, fp, C.c_uint32, fp]
+            getattr(L, f"oracle_material_math_{sfx}").argtypes = [C.c_int, fp]
+            getattr(L, f"oracle_material_math_{sfx}").restype = (
+                C.c_float if sfx == "f32" else C.c_double)
         L.oracle_onb_f32.argtypes = [f32p, f32p, f32p]
         L.oracle_aces_f32.argtypes = [f32p, f32p]
         L.oracle_tex_bilinear_f32.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_float,
@@ -261,6 +270,52 @@ def disney_sample(mat, Ns, Ng, V, seed, f64=False):
     getattr(lib(), f"oracle_disney_sample_{sfx}")(m.ctypes.data, *[_p(x, fp) for x in a],
                                                   C.byref(s), _p(out, fp))
     return out, s.value
+
+
+def disney_sample_batch(mat, Ns, Ng, V, seeds, f64=True):
+    """Directions (n, 3) sampled from the seeds (n,), one per seed."""
+    dt, fp, sfx = (np.float64, f64p, "f64") if f64 else (np.float32, f32p, "f32")
+    m = np.ascontiguousarray(mat, np.float32).reshape(16)
+    a = [_v(x, dt) for x in (Ns, Ng, V)]
+    sd = np.ascontiguousarray(seeds, np.uint32).reshape(-1)
+    out = np.zeros((sd.size, 3), dt)
+    getattr(lib(), f"oracle_disney_sample_batch_{sfx}")(
+        m.ctypes.data, *[_p(x, fp) for x in a], _p(sd, u32p), sd.size, _p(out, fp))
+    return out
+
+
+def disney_pdf_batch(mat, Ns, Ng, V, L, f64=True):
+    dt, fp, sfx = (np.float64, f64p, "f64") if f64 else (np.float32, f32p, "f32")
+    m = np.ascontiguousarray(mat, np.float32).reshape(16)
+    a = [_v(x, dt) for x in (Ns, Ng, V)]
+    Ls = np.ascontiguousarray(L, dt).reshape(-1, 3)
+    out = np.zeros(Ls.shape[0], dt)
+    getattr(lib(), f"oracle_disney_pdf_batch_{sfx}")(
+        m.ctypes.data, *[_p(x, fp) for x in a], _p(Ls, fp), Ls.shape[0], _p(out, fp))
+    return out
+
+
+def disney_eval_batch(mat, Ns, Ng, V, L, f64=True):
+    dt, fp, sfx = (np.float64, f64p, "f64") if f64 else (np.float32, f32p, "f32")
+    m = np.ascontiguousarray(mat, np.float32).reshape(16)
+    a = [_v(x, dt) for x in (Ns, Ng, V)]
+    Ls = np.ascontiguousarray(L, dt).reshape(-1, 3)
+    out = np.zeros((Ls.shape[0], 3), dt)
+    getattr(lib(), f"oracle_disney_eval_batch_{sfx}")(
+        m.ctypes.data, *[_p(x, fp) for x in a], _p(Ls, fp), Ls.shape[0], _p(out, fp))
+    return out
+
+
+MM_SCHLICK, MM_FRESNEL, MM_GTR1, MM_GTR2, MM_SMITHG, MM_LAMBDA = range(6)
+
+
+def material_math(which, *args, f64=True):
+    """One MaterialMath function (Common.h:119-181) of the oracle."""
+    dt, fp, sfx = (np.float64, f64p, "f64") if f64 else (np.float32, f32p, "f32")
+    a = np.zeros(8, dt)
+    flat = np.concatenate([np.atleast_1d(np.asarray(x, dt)).reshape(-1) for x in args])
+    a[:flat.size] = flat
+    return getattr(lib(), f"oracle_material_math_{sfx}")(which, _p(a, fp))
 
 
 def onb(n):

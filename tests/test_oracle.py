@@ -14,9 +14,12 @@ Pinning status per oracle function
                            dielectric -> Burley 2012 diffuse + GGX; symmetry
   disney pdf / sample      pdf integrates to <= 1 (~1 for rough opaque);
                            sampler histogram vs pdf (opaque, clearcoat 0)
-  disney eval (transmit)   positivity and finiteness only: parity unpinned
-                           (no closed form; the reference's Beckmann-lambda /
-                           GTR2 mix has no textbook counterpart)
+  disney eval (transmit),  pinned in test_oracle_pins.py: smooth-surface
+  sample/pdf (refraction,  limit against Snell + Fresnel; reversal identity;
+  TIR), MaterialMath,      microfacet normals of refracted samples vs GGX;
+  Russian roulette,        sampler vs pdf chi-square on both sides of the
+  pinhole mapping          glass; Fresnel/Lambda/GTR/Smith vs textbook forms;
+                           RR vs the RR-free estimate; emitter projection
   light sample/pdf + MIS   UPT, NEE and MIS estimators agree in expectation
   integrator               direct view of an emitter returns L exactly; miss
                            returns the background; FP32 and FP64 agree
