@@ -342,7 +342,10 @@ wl_status wl_guide_get_info(wl_guide guide, wl_guide_info *info);
 wl_status wl_guide_infer(wl_guide guide, const float *d_queries, uint32_t n, float *d_out,
                          void *stream);
 /* Mixture sampling on given outputs: d_u = 3 uniforms per query (lobe, then
- * the two direction numbers); writes the direction and its mixture density. */
+ * the two direction numbers); writes the direction and its mixture density.
+ * Both calls run the path tracer's own decode chain (the outputs decoded
+ * across lane groups into the stored per-query form, read back, sampled with
+ * the stored lobe weights), so they test what a guided vertex does. */
 wl_status wl_guide_mixture_sample(wl_context ctx, const float *d_out, const float *d_u,
                                   uint32_t n, float *d_dirs, float *d_pdf, void *stream);
 wl_status wl_guide_mixture_pdf(wl_context ctx, const float *d_out, const float *d_dirs,
@@ -365,7 +368,8 @@ wl_status wl_scene_set_guide(wl_scene scene, wl_guide guide, uint32_t mode);
  * graph is replayed per megabatch, reading its work list from device memory.
  * Each request's film equals wl_pt_render of the same request bit for bit.
  * Requests may use different scenes of the server's device; their scenes must
- * have no guide attached. */
+ * have no guide attached, and their films must not overlap (rejected with
+ * WL_ERROR_INVALID_VALUE: the requests' films are updated concurrently). */
 typedef struct wl_render_request
 {
     wl_scene scene;

@@ -118,38 +118,56 @@ void k_guide_infer(const bf16x8 *__restrict__ fwd,
 }
 
 // ---- mixture sampling / density on given network outputs -----------------------
-__global__ void k_guide_mix_sample(const float *__restrict__ outs, const float *__restrict__ u,
-                                   uint32_t n, float *__restrict__ dirs, float *__restrict__ pdf)
+// The same decode chain as the path tracer's guided vertices: the outputs are
+// placed in the lane layout guide_forward leaves them in (lane group g holds
+// lobes 2 g and 2 g + 1 of 16 queries), decoded across the lane groups into
+// the stored form (guide_decode_lanes, here into LDS instead of the path
+// tracer's per-query buffer), then read back with guide_load and sampled
+// with the stored lobe weights.
+template <bool SAMPLE>
+__global__ __launch_bounds__(kGuideBlock)
+void k_guide_mix(const float *__restrict__ outs, const float *__restrict__ in, uint32_t n,
+                 float *__restrict__ dirs, float *__restrict__ pdf)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    __shared__ float dec[(kGuideBlock / 64) * 16 * kGuideDecoded];
+    const int lane = threadIdx.x & 63, g = lane >> 4, wave = threadIdx.x / 64;
+    float *wdec = dec + wave * 16 * kGuideDecoded;
+    const uint32_t tiles = (n + 15) / 16;
+    const uint32_t per_block = kGuideBlock / 64;
+    // block-uniform loop: every wave reaches the barriers equally often
+    for (uint32_t t0 = blockIdx.x * per_block; t0 < tiles; t0 += gridDim.x * per_block)
     {
-        float o[kGuideOut];
+        const uint32_t i = (t0 + wave) * 16 + (lane & 15);
+        const bool valid = i < n;
+        floatx4 o[2];
 #pragma unroll
-        for (int r = 0; r < kGuideOut; ++r)
-            o[r] = outs[32 * (size_t)i + r];
-        GuideMix m;
-        guide_decode(o, m);
-        const V3 w = guide_sample(m, nullptr, u[3 * (size_t)i], u[3 * (size_t)i + 1],
-                                  u[3 * (size_t)i + 2]);
-        dirs[3 * (size_t)i] = w.x;
-        dirs[3 * (size_t)i + 1] = w.y;
-        dirs[3 * (size_t)i + 2] = w.z;
-        pdf[i] = guide_pdf(m, w);
-    }
-}
-
-__global__ void k_guide_mix_pdf(const float *__restrict__ outs, const float *__restrict__ dirs,
-                                uint32_t n, float *__restrict__ pdf)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        float o[kGuideOut];
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int r = 0; r < kGuideOut; ++r)
-            o[r] = outs[32 * (size_t)i + r];
-        GuideMix m;
-        guide_decode(o, m);
-        pdf[i] = guide_pdf(m, mk(dirs[3 * (size_t)i], dirs[3 * (size_t)i + 1], dirs[3 * (size_t)i + 2]));
+            for (int c = 0; c < 4; ++c)
+                o[t][c] = valid ? outs[32 * (size_t)i + 16 * t + 4 * g + c] : 0.0f;
+        guide_decode_lanes(o, g, wdec + kGuideDecoded * (lane & 15), valid);
+        __syncthreads();
+        if (valid && lane < 16)
+        {
+            const float *src = wdec + kGuideDecoded * lane;
+            GuideMix m;
+            guide_load(src, m);
+            V3 w;
+            if (SAMPLE)
+            {
+                w = guide_sample(m, src + 40, in[3 * (size_t)i], in[3 * (size_t)i + 1],
+                                 in[3 * (size_t)i + 2]);
+                dirs[3 * (size_t)i] = w.x;
+                dirs[3 * (size_t)i + 1] = w.y;
+                dirs[3 * (size_t)i + 2] = w.z;
+            }
+            else
+            {
+                w = mk(in[3 * (size_t)i], in[3 * (size_t)i + 1], in[3 * (size_t)i + 2]);
+            }
+            pdf[i] = guide_pdf(m, w);
+        }
+        __syncthreads();
     }
 }
 
@@ -486,8 +504,9 @@ hipError_t guide_mixture_sample(const float *outs, const float *u, uint32_t n, f
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_guide_mix_sample, dim3(std::min<uint32_t>((n + 255) / 256, 4096)),
-                       dim3(256), 0, s, outs, u, n, dirs, pdf);
+    const uint32_t tiles = (n + 15) / 16;
+    hipLaunchKernelGGL(k_guide_mix<true>, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)),
+                       dim3(kGuideBlock), 0, s, outs, u, n, dirs, pdf);
     return hipGetLastError();
 }
 
@@ -496,8 +515,9 @@ hipError_t guide_mixture_pdf(const float *outs, const float *dirs, uint32_t n, f
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_guide_mix_pdf, dim3(std::min<uint32_t>((n + 255) / 256, 4096)),
-                       dim3(256), 0, s, outs, dirs, n, pdf);
+    const uint32_t tiles = (n + 15) / 16;
+    hipLaunchKernelGGL(k_guide_mix<false>, dim3(std::min<uint32_t>((tiles + 3) / 4, 4096)),
+                       dim3(kGuideBlock), 0, s, outs, dirs, n, nullptr, pdf);
     return hipGetLastError();
 }
 

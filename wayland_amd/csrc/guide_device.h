@@ -155,30 +155,6 @@ struct GuideMix
     V3 mu[kGuideLobes];
 };
 
-// o: the 32 outputs in row order.
-__device__ __forceinline__ void guide_decode(const float o[kGuideOut], GuideMix &m)
-{
-    float amax = -3.0e38f;
-#pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j)
-        amax = fmaxf(amax, o[guide_row(j, 0)]);
-    float s = 0.f;
-#pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j)
-        s += expf(o[guide_row(j, 0)] - amax);
-    const float lse = amax + logf(s);
-#pragma unroll
-    for (int j = 0; j < kGuideLobes; ++j)
-    {
-        m.kappa[j] = guide_softplus(o[guide_row(j, 1)]);
-        m.lw[j] = (o[guide_row(j, 0)] - lse) + vmf_log_norm(m.kappa[j]);
-        float st, ct, sp, cp;
-        sincosf(o[guide_row(j, 2)], &st, &ct);
-        sincosf(o[guide_row(j, 3)], &sp, &cp);
-        m.mu[j] = mk(st * cp, st * sp, ct);
-    }
-}
-
 // Decoded mixture as the path tracer stores it per query (48 floats):
 // float4 (lw, kappa, mu.x, mu.y) for lobes 0..7, then mu.z[8], then the lobe
 // weights pi[8].

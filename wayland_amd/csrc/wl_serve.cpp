@@ -229,6 +229,18 @@ wl_status validate_requests(wl_server s, const wl_serve_config &cfg,
             return set_error(WL_ERROR_LIMIT_EXCEEDED,
                              "batch_paths must hold the largest request's pixel count");
     }
+    // Films must not overlap: one megabatch (or two lanes at once) updates
+    // every request's film, so two requests sharing pixels would race.
+    std::vector<std::pair<uintptr_t, uintptr_t>> span(n);
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        const uintptr_t b = reinterpret_cast<uintptr_t>(reqs[k].film);
+        span[k] = { b, b + sizeof(float) * 6u * (uintptr_t)reqs[k].width * reqs[k].height };
+    }
+    std::sort(span.begin(), span.end());
+    for (uint32_t k = 1; k < n; ++k)
+        if (span[k].first < span[k - 1].second)
+            return set_error(WL_ERROR_INVALID_VALUE, "two requests' films overlap");
     return WL_SUCCESS;
 }
 
@@ -714,6 +726,17 @@ wl_status wl_server_autotune(wl_server s, const wl_render_request *reqs, uint32_
     }
     hipFree(scratch);
     s->cfg = st == WL_SUCCESS && best_rate > 0 ? best : saved;
+    // the candidates left the largest batch buffers behind: keep only what
+    // the chosen configuration renders with (the twin lane only for lanes 2)
+    if (s->twin && s->cfg.lanes != 2)
+    {
+        wl_server_destroy(s->twin);
+        s->twin = nullptr;
+    }
+    if (st == WL_SUCCESS)
+        st = ensure_buffers(s, s->cfg.batch_paths);
+    if (st == WL_SUCCESS && s->twin)
+        st = ensure_buffers(s->twin, s->cfg.batch_paths);
     if (chosen)
         *chosen = s->cfg;
     return st;
