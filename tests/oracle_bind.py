@@ -118,6 +118,9 @@ def lib():
                                                  C.POINTER(C.c_uint64)]
         L.oracle_pt_render_f64.argtypes = [C.c_void_p, C.POINTER(PTConfig), C.c_uint32,
                                            C.c_uint32, f64p, C.c_int]
+        L.oracle_pt_render_f64_window.argtypes = [
+            C.c_void_p, C.POINTER(PTConfig), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+            C.c_uint32, C.c_uint32, f64p, f64p, C.c_int]
         L.oracle_film_update.argtypes = [f32p, u8p, f32p, C.c_uint32, C.c_uint32]
         L.oracle_example_render.argtypes = [C.c_void_p, f32p, f32p, C.POINTER(ExampleConfig),
                                             C.c_uint32, C.c_uint32, f32p, C.c_int]
@@ -409,6 +412,17 @@ class OraclePT:
         lib().oracle_pt_render_f64(self.h, C.byref(cfg), frame0, nframes, _p(out, f64p),
                                    nthreads or nthreads_default())
         return out
+
+    def render_f64_window(self, cfg, x0, y0, ww, wh, frame0, nframes, nthreads=None):
+        """FP64 mean and per-pixel one-sample variance over a window of the
+        cfg.width x cfg.height image (pixels keep their whole-image seeds)."""
+        assert x0 + ww <= cfg.width and y0 + wh <= cfg.height
+        mean = np.zeros((wh, ww, 3), np.float64)
+        var = np.zeros((wh, ww, 3), np.float64)
+        lib().oracle_pt_render_f64_window(self.h, C.byref(cfg), x0, y0, ww, wh, frame0,
+                                          nframes, _p(mean, f64p), _p(var, f64p),
+                                          nthreads or nthreads_default())
+        return mean, var
 
     def example_render(self, mesh_color, mesh_emission, ecfg, row0, row1, nthreads=None):
         mc = np.ascontiguousarray(mesh_color, np.float32)
