@@ -277,6 +277,23 @@ wl_status wl_pt_render_samples(wl_scene scene, const wl_pt_config *cfg,
 wl_status wl_pt_launch_frame(wl_scene scene, const wl_pt_config *cfg,
                              uint32_t frame, float *d_radiance_rgba,
                              uint8_t *d_color_rgba, void *stream);
+/* wl_pt_render without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The batch
+ * buffers (path state, ray / hit / shadow queues, queue counters) live in one
+ * fixed device array of the library, sized for WL_PT_NOALLOC_MAX_BATCH_PATHS
+ * paths per batch (about 61 MB, held on each device the library runs on
+ * until the process ends); the call makes no hipMalloc / hipFree and no
+ * memset node.  Batches are formed as in wl_pt_render (one lane, no stats):
+ * cfg->max_batch_paths = 0 means WL_PT_NOALLOC_MAX_BATCH_PATHS, a larger value
+ * is rejected with WL_ERROR_LIMIT_EXCEEDED, and a scene with a guide attached
+ * with WL_ERROR_NOT_SUPPORTED.  The film is the one wl_pt_render writes for
+ * the same cfg, bit for bit.  Every call on one device shares that array:
+ * calls that overlap on one GPU (different streams or threads) race, so order
+ * them on one stream.  Needs a context created on the device first. */
+#define WL_PT_NOALLOC_MAX_BATCH_PATHS (1u << 18)
+wl_status wl_pt_render_noalloc(wl_scene scene, const wl_pt_config *cfg, float *d_film,
+                               void *stream);
 /* Film resolve: d_rgb[i] = sum / nsamples (+ compensation). */
 wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
                           uint32_t nsamples, float *d_rgb, void *stream);

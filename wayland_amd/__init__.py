@@ -29,7 +29,8 @@ HIT_DTYPE = np.dtype([("t", np.float32), ("prim", np.uint32),
 
 STRATEGY_UPT, STRATEGY_NEE, STRATEGY_MIS = 0, 1, 2
 PT_SORT_BY_MATERIAL = 1
-PT_SORT_BY_MATERIAL = 1
+# paths per batch of PathTracer.render_noalloc (WL_PT_NOALLOC_MAX_BATCH_PATHS)
+NOALLOC_MAX_BATCH_PATHS = 1 << 18
 
 
 def _torch():
@@ -549,6 +550,14 @@ class PathTracer:
                   C.byref(st) if stats else None, _stream_ptr(film.device))
         return film, ({k: getattr(st, k) for k, _ in st._fields_ if k != "pad_"}
                       if stats else None)
+
+    def render_noalloc(self, cfg, film):
+        """wl_pt_render_noalloc: the same film as render(), from the library's
+        fixed scratch array (no allocation; batches of at most
+        NOALLOC_MAX_BATCH_PATHS paths)."""
+        _lib.call("wl_pt_render_noalloc", self.h, C.byref(cfg), C.c_void_p(film.data_ptr()),
+                  _stream_ptr(film.device))
+        return film
 
     def render_samples(self, cfg, frame, out=None):
         torch = _torch()

@@ -181,6 +181,7 @@ SIGNATURES = {
     "wl_scene_set_textures": (C.c_int, [_VP, C.POINTER(TextureImage), _U32,
                                         C.POINTER(MeshTexture), _U32]),
     "wl_pt_render": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, C.POINTER(PTStats), _VP]),
+    "wl_pt_render_noalloc": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, _VP]),
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
     "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),
     "wl_film_resolve": (C.c_int, [_VP, _VP, _U32, _U32, _VP, _VP]),

@@ -33,6 +33,7 @@ SOURCES = [
     "guide.hip",
     "pathtrace.hip",
     "example_pt.hip",
+    "wl_pt_noalloc.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -105,6 +105,14 @@ wl_status wl_context_create(int device, wl_context *out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess)
         ctx->num_cus = prop.multiProcessorCount;
+    // per-device set-up of wl_pt_render_noalloc's fixed scratch array
+    const wl_status st = wl::noalloc_setup(device);
+    if (st != WL_SUCCESS)
+    {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return st;
+    }
     *out = ctx;
     return WL_SUCCESS;
 }

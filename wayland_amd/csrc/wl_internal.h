@@ -122,6 +122,8 @@ size_t pt_batch_bytes(size_t cap);
 PtScene pt_scene_view(wl_scene s);
 wl_status pt_check_config(wl_scene s, const wl_pt_config *c);
 PtParams pt_params_for(wl_scene s, const wl_pt_config *c);
+// looks up this device's no-allocation scratch array (wl_pt_noalloc.hip)
+wl_status noalloc_setup(int device);
 // Adam steps of a guide over samples[0, min(*n_dev, n_cap)) (wl_guide_api.cpp)
 wl_status guide_train_device(wl_guide g, const float *samples, const uint32_t *n_dev,
                              uint32_t n_cap, uint32_t steps, hipStream_t s);
