@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--scene", default="25x6", help="COUNTxSUBDIV icospheres")
     p.add_argument("--rays", type=int, default=1 << 23)
+    p.add_argument("--scene-b", default="64x7",
+                   help="second traversal point, COUNTxSUBDIV ('' = skip)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (PMC scale factor)")
@@ -49,7 +51,7 @@ def parse():
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
     p.add_argument("--guide-ref-spp", type=int, default=4096)
-    p.add_argument("--guide-runs", type=int, default=3)
+    p.add_argument("--guide-runs", type=int, default=10)
     p.add_argument("--serve-requests", type=int, default=64,
                    help="batched-serving requests (0 = skip)")
     p.add_argument("--scene-file-frames", type=int, default=64,
@@ -58,6 +60,46 @@ def parse():
                    help="FP64 oracle samples per pixel for the Cornell relMSE "
                         "(0 = skip the Cornell leg)")
     return p.parse_args()
+
+
+def bench_traversal_point(args, ctx, wl, torch, timed):
+    """The traversal workload's other point (default 64 icospheres at
+    subdivision 7 = 20,971,520 triangles) with the same 2^23 rays:
+    closest-hit and any-hit Grays/s and the algorithmic-byte roofline."""
+    from wayland_amd import scenes
+    count, subdiv = (int(x) for x in args.scene_b.split("x"))
+    t0 = time.perf_counter()
+    sc = scenes.icosphere_scene(count, subdiv)
+    t_gen = time.perf_counter() - t0
+    accel = wl.Accel(ctx, sc)
+    info = accel.info()
+    rays = torch.from_numpy(scenes.traversal_rays(args.rays, seed=2)).cuda()
+    n = rays.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    bits = torch.empty(((n + 31) // 32,), dtype=torch.int32, device="cuda")
+    t_c = timed(lambda: accel.trace_closest(rays, hits), args.steps, args.warmup)
+    t_a = timed(lambda: accel.trace_any(rays, bits), args.steps, args.warmup)
+    sub = rays[::64].contiguous()
+    cnt = torch.empty((sub.shape[0], 2), dtype=torch.int32, device="cuda")
+    accel.trace_closest(sub, None, cnt)
+    torch.cuda.synchronize()
+    c = cnt.double().mean(0).cpu().numpy()
+    bpr = RAY_BYTES + HIT_BYTES + c[0] * NODE_BYTES + c[1] * TRI_BYTES
+    out = {"workload": f"{count} displaced icospheres subdiv {subdiv} "
+                       f"({sc.num_triangles} tris), {n} rays",
+           "triangles": sc.num_triangles, "closest_grays_per_s": n / t_c / 1e9,
+           "anyhit_grays_per_s": n / t_a / 1e9, "ms_closest": t_c * 1e3,
+           "bvh": {"nodes": info["inner_nodes"], "depth": info["depth"],
+                   "build_s": info["build_seconds"], "device_bytes": info["device_bytes"]},
+           "scene_gen_s": t_gen,
+           "roofline": {"bound": "hbm", "achieved": bpr * n / t_c / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": bpr * n / t_c / 1e9 / HBM_PEAK_GBS,
+                        "bytes_per_ray": bpr, "nodes_per_ray": c[0], "tris_per_ray": c[1]}}
+    log(f"traversal point b: {out}")
+    accel.close()
+    del rays, hits, bits
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_pt(args, ctx, accel, sc, wl, torch):
@@ -177,9 +219,21 @@ def bench_guide(args, ctx, wl, torch):
     progressive(1, 1 << 28)
     pt.set_guide(None)
     warm_guide.close()
+    # the converged FP64 oracle crop (65,536 spp) at the caustic, when the
+    # view is the fixture's
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fp64_references as fx
+    crop = fx.load("caustic_crop128_fp64")
+    if crop is not None and crop[2]["width"] != res:
+        crop = None
+
+    def crop_of(img):
+        x0, y0, ww, wh = crop[2]["window"]
+        return img.reshape(res, res, 3)[y0:y0 + wh, x0:x0 + ww]
+
     # independent runs: fresh network each time, training inside the timing
     runs = args.guide_runs
-    g_rel, g_trim, g_floor, g_t, steps = [], [], [], [], 0
+    g_rel, g_trim, g_floor, g_crop, g_t, steps = [], [], [], [], [], 0
     for r in range(runs):
         guide = wl.Guide(ctx, seed=r + 1)
         pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
@@ -190,6 +244,8 @@ def bench_guide(args, ctx, wl, torch):
         g_rel.append(relmse(img_g, ref))
         g_trim.append(relmse(img_g, ref, 1e-3))
         g_floor.append(relmse(img_g, ref, mask=floor))
+        if crop is not None:
+            g_crop.append(relmse(crop_of(img_g), crop[0]))
         g_t.append(t_g)
     # plain PT per-sample time (after a warm pass), then the same wall time as
     # the guided runs
@@ -197,12 +253,14 @@ def bench_guide(args, ctx, wl, torch):
     _, t_probe = progressive(16, 1 << 26)
     spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 16))))
     for attempt in range(3):
-        p_rel, p_trim, p_floor, p_t = [], [], [], []
+        p_rel, p_trim, p_floor, p_crop, p_t = [], [], [], [], []
         for r in range(runs):
             img_p, t_p = progressive(spp_p, (1 << 27) + (r << 20))
             p_rel.append(relmse(img_p, ref))
             p_trim.append(relmse(img_p, ref, 1e-3))
             p_floor.append(relmse(img_p, ref, mask=floor))
+            if crop is not None:
+                p_crop.append(relmse(crop_of(img_p), crop[0]))
             p_t.append(t_p)
         # equal time within 3 %, else rescale the sample count and rerun
         ratio = float(np.mean(g_t)) / float(np.mean(p_t))
@@ -210,20 +268,48 @@ def bench_guide(args, ctx, wl, torch):
             break
         spp_p = max(1, int(round(spp_p * ratio)))
     out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
-           "reference": f"plain PT {ref_spp} spp on the GPU",
            "runs": runs,
-           "floor_pixels": int(floor.sum()),
-           "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)),
-                      "relmse": g_rel, "relmse_trim_0.1pct": g_trim, "relmse_floor": g_floor,
-                      "adam_steps": steps},
-           "plain": {"spp": spp_p, "seconds": float(np.mean(p_t)), "relmse": p_rel,
-                     "relmse_trim_0.1pct": p_trim, "relmse_floor": p_floor},
-           "relmse_floor_ratio_equal_time": float(np.mean(g_floor) / np.mean(p_floor)),
-           "relmse_ratio_equal_time": float(np.mean(g_rel) / np.mean(p_rel)),
-           "relmse_median_ratio_equal_time": float(np.median(g_rel) / np.median(p_rel)),
-           "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim))}
+           "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)), "adam_steps": steps},
+           "plain": {"spp": spp_p, "seconds": float(np.mean(p_t))}}
+    if crop is not None:
+        x0, y0, ww, wh = crop[2]["window"]
+        out["caustic_crop"] = {
+            "reference": f"FP64 CPU oracle, {crop[2]['spp']} spp, window x {x0}..{x0 + ww - 1}, "
+                         f"y {y0}..{y0 + wh - 1} of the {res}^2 view (the caustic on the floor)",
+            "relmse_guided": g_crop, "relmse_plain": p_crop,
+            "relmse_ratio_equal_time": float(np.mean(g_crop) / np.mean(p_crop)),
+            "relmse_ratio_ci95": bootstrap_ratio_ci(g_crop, p_crop),
+            "relmse_plain_expected": fx.expected_relmse(crop[0], crop[1], spp_p)}
+    out["full_image"] = {
+        "reference": f"plain PT {ref_spp} spp on the GPU (not converged: fireflies of "
+                     "glass-only paths)",
+        "floor_pixels": int(floor.sum()),
+        "relmse_guided": g_rel, "relmse_plain": p_rel,
+        "relmse_trim_0.1pct_guided": g_trim, "relmse_trim_0.1pct_plain": p_trim,
+        "relmse_floor_guided": g_floor, "relmse_floor_plain": p_floor,
+        "relmse_ratio_equal_time": float(np.mean(g_rel) / np.mean(p_rel)),
+        "relmse_ratio_ci95": bootstrap_ratio_ci(g_rel, p_rel),
+        "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim)),
+        "relmse_floor_ratio_equal_time": float(np.mean(g_floor) / np.mean(p_floor))}
+    # headline: the crop against the converged FP64 oracle where present
+    head = out.get("caustic_crop", out["full_image"])
+    out["relmse_ratio_equal_time"] = head["relmse_ratio_equal_time"]
+    out["relmse_ratio_ci95"] = head["relmse_ratio_ci95"]
     log(f"guide: {out}")
     return out
+
+
+def bootstrap_ratio_ci(a, b, n=10000, seed=0):
+    """95 % percentile bootstrap interval of mean(a) / mean(b), resampling the
+    independent runs of each side with replacement."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    if a.size < 2 or b.size < 2:
+        return None
+    rng = np.random.default_rng(seed)
+    ra = a[rng.integers(0, a.size, (n, a.size))].mean(1)
+    rb = b[rng.integers(0, b.size, (n, b.size))].mean(1)
+    q = np.percentile(ra / rb, [2.5, 97.5])
+    return [float(q[0]), float(q[1])]
 
 
 def bench_serve(args, ctx, wl, torch):
@@ -234,21 +320,12 @@ def bench_serve(args, ctx, wl, torch):
     same requests rendered one after another with wl_pt_render."""
     from wayland_amd import scenes
     rng = np.random.default_rng(5)
-    tracers = []
-    for k in range(4):
-        sc = scenes.cornell_box(256, 256, seed=k)
-        tracers.append(wl.PathTracer(ctx, wl.Accel(ctx, sc), sc))
-    for k in range(4):
-        sc = scenes.wavefront_scene(8, 4, seed=10 + k, width=256, height=256)
-        tracers.append(wl.PathTracer(ctx, wl.Accel(ctx, sc), sc))
+    tracers = [wl.PathTracer(ctx, wl.Accel(ctx, sc), sc) for sc in scenes.serving_scenes()]
+
     def workload(n, res_lo, res_hi, spp_hi, seed_shift):
-        reqs = []
-        for k in range(n):
-            res = int(rng.integers(res_lo, res_hi + 1))
-            reqs.append({"pt": tracers[int(rng.integers(0, len(tracers)))], "width": res,
-                         "height": res, "spp": int(rng.integers(1, spp_hi + 1)),
-                         "max_depth": int(rng.integers(8, 17)),
-                         "frame_offset": (k + seed_shift) << 16})
+        reqs = scenes.serving_requests(rng, n, res_lo, res_hi, spp_hi, seed_shift)
+        for r in reqs:
+            r["pt"] = tracers[r.pop("scene")]
         return reqs
 
     def run(reqs, label):
@@ -271,6 +348,7 @@ def bench_serve(args, ctx, wl, torch):
         t_seq = time.perf_counter() - t
         done = np.array(done)
         out = {"workload": label, "paths": paths, "megabatches": st["batches"],
+               "graph_launches": st["launches"],
                "autotuned": tuned, "autotune_seconds": t_tune,
                "mpaths_per_s": paths / st["seconds"] / 1e6, "seconds": st["seconds"],
                "grays_per_s": (st["extension_rays"] + st["shadow_rays"]) / st["seconds"] / 1e9,
@@ -312,7 +390,8 @@ def bench_cornell(args, ctx, wl, torch):
         times.append(time.perf_counter() - t)
     t_gpu = float(np.median(times))
     img_gpu = pt.resolve(film, spp).cpu().numpy().reshape(res, res, 3)
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    model, ncpu, avail = host_cpu()
+    nthreads = max(1, min(HOST_SHARE_THREADS, avail))
     opt = ob.OraclePT(sc)
     ocfg = opt.config(res, res, max_depth=depth)
     opt.render_frame(ocfg, 0, nthreads)  # warm
@@ -323,19 +402,36 @@ def bench_cornell(args, ctx, wl, torch):
         acc += img
     t_cpu = time.perf_counter() - t
     img_cpu = acc / spp
-    t = time.perf_counter()
-    ref = opt.render_f64(ocfg, 1 << 20, args.cornell_ref_spp, nthreads)
-    t_ref = time.perf_counter() - t
+    # the converged reference: the committed 65,536-spp FP64 oracle image of
+    # this workload; without it, an FP64 oracle render of cornell_ref_spp
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fp64_references as fx
+    fixture = fx.load("cornell_256_fp64")
+    expected = None
+    if fixture is not None:
+        ref, ref_var, meta = fixture
+        ref_spp, t_ref = meta["spp"], meta["seconds"]
+        ref_kind = f"FP64 CPU oracle, {ref_spp} spp (committed fixture, {meta['threads']} threads)"
+        expected = fx.expected_relmse(ref, ref_var, spp, ref_spp)
+    else:
+        t = time.perf_counter()
+        ref_spp = args.cornell_ref_spp
+        ref = opt.render_f64(ocfg, 1 << 20, ref_spp, nthreads)
+        t_ref = time.perf_counter() - t
+        ref_kind = f"FP64 CPU oracle, {ref_spp} spp"
     paths = res * res * spp
     out = {"workload": f"cornell box {res}x{res}, {spp} spp, depth {depth}, NEE+MIS, FP32, "
                        "TEA/LCG seeds (pixel, frame) as PathTracing.cu",
            "gpu_mpaths_per_s": paths / t_gpu / 1e6, "gpu_seconds": t_gpu,
            "cpu_mpaths_per_s": paths / t_cpu / 1e6, "cpu_seconds": t_cpu,
-           "cpu_threads": nthreads, "cpu_kind": "port",
+           "cpu_threads": nthreads, "cpu_kind": "port", "cpu_model": model,
+           "host_cpus": ncpu,
            "relmse_gpu_vs_fp64_oracle": relmse(img_gpu, ref),
            "relmse_cpu_vs_fp64_oracle": relmse(img_cpu, ref),
+           "relmse_expected_at_16spp": expected,
            "max_abs_gpu_minus_cpu": float(np.abs(img_gpu - img_cpu).max()),
-           "fp64_oracle_spp": args.cornell_ref_spp, "fp64_oracle_seconds": t_ref}
+           "fp64_oracle": ref_kind, "fp64_oracle_spp": ref_spp,
+           "fp64_oracle_seconds": t_ref}
     log(f"cornell: {out}")
     return out
 
@@ -402,20 +498,49 @@ def traffic_from_profiles(tag):
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """(model string, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+# The host share of one GPU on the measurement boxes is 16 CPUs: the CPU
+# baseline uses at most that many threads, whatever the host reports.
+HOST_SHARE_THREADS = 16
+
+
 def cpu_baseline(tris, rays_np):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    model, ncpu, avail = host_cpu()
+    nthreads = max(1, min(HOST_SHARE_THREADS, avail))
     bvh = ob.OracleBVH(tris)
     sample = rays_np
     bvh.closest(sample[:4096], nthreads)  # warm
-    t0 = time.perf_counter()
-    bvh.closest(sample, nthreads)
-    dt = time.perf_counter() - t0
+    # repeat the whole ray set until about 10 s of CPU time (threads x wall)
+    # is spent; the rate is the median over the repetitions
+    times = []
+    while not times or (sum(times) * nthreads < 10.0 and len(times) < 50):
+        t0 = time.perf_counter()
+        bvh.closest(sample, nthreads)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return {"value": sample.shape[0] / dt / 1e9, "unit": "Grays/s", "cores": nthreads,
-            "kind": "port",
+            "kind": "port", "cpu_model": model, "host_cpus": ncpu, "cpus_available": avail,
             "sample": f"oracle BVH2 watertight closest-hit, all {sample.shape[0]} rays "
-                      f"of the workload, {nthreads} threads, {dt:.2f} s"}
+                      f"of the workload, {nthreads} threads (the host share of one GPU), "
+                      f"median of {len(times)} passes of {dt:.2f} s"}
 
 
 def main():
@@ -481,6 +606,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
 
+    point_b = None
+    if args.scene_b:
+        point_b = bench_traversal_point(args, ctx, wl, torch, timed)
+
     pt_out = None
     if args.pt_spp > 0:
         pt_out = bench_pt(args, ctx, accel, sc, wl, torch)
@@ -530,9 +659,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     # measured beyond-L2 bytes per launch over the same time:
+                     # the share of HBM bandwidth the launch actually draws
+                     "traffic_frac": (traffic / t_closest / 1e9 / HBM_PEAK_GBS
+                                      if traffic else None),
                      "bytes_per_ray": bytes_per_ray, "nodes_per_ray": c[0],
                      "tris_per_ray": c[1]},
         "cpu_baseline": cpu,
+        "traversal_point_b": point_b,
         "pathtracer": pt_out,
         "guiding": guide_out,
         "cornell_cpu_workload": cornell_out,

@@ -271,6 +271,16 @@ wl_status wl_pt_render(wl_scene scene, const wl_pt_config *cfg, float *d_film,
  * (pixel, frame)): d_rgb = W*H*3 floats.  For parity with the reference. */
 wl_status wl_pt_render_samples(wl_scene scene, const wl_pt_config *cfg,
                                uint32_t frame, float *d_rgb, void *stream);
+/* wl_pt_render_samples of one batch (W*H <= max_batch_paths, no guide) with
+ * a per-vertex record of every path, for parity diagnosis against the
+ * oracle's per-vertex trace: d_trace = W*H*max_vertices*8 floats, zeroed by
+ * the caller; path p's vertex of depth k (the k-th surface vertex, counted
+ * from 1) that sampled a direction writes 8 floats at ((p*max_vertices)+k-1)*8:
+ * depth, mesh, pdf, mean f|cos|/pdf, mean throughput reaching the vertex, 0,
+ * 0, trans.  Vertices beyond max_vertices are not recorded. */
+wl_status wl_pt_trace_samples(wl_scene scene, const wl_pt_config *cfg, uint32_t frame,
+                              float *d_rgb, float *d_trace, uint32_t max_vertices,
+                              void *stream);
 /* One progressive frame exactly as the reference raygen ends
  * (PathTracing.cu:145-158): running mean into d_radiance (RGBA float) with
  * float factors and double temporaries, then ACES (Tonemap.h:34-42) -> u8. */
@@ -381,8 +391,9 @@ wl_status wl_scene_set_guide(wl_scene scene, wl_guide guide, uint32_t mode);
  * (src/Frontend/Core/DeviceManager.cpp:231-241, Launcher.cpp:12-20) for a
  * set of independent render requests.  The server packs whole-image sample
  * passes of the requests (in order) into megabatches of up to batch_paths
- * paths; a megabatch's kernels are captured once into a hipGraph and the
- * graph is replayed per megabatch, reading its work list from device memory.
+ * paths; a call's whole megabatch sequence is captured into one hipGraph and
+ * launched once per lane, each megabatch reading its work list from device
+ * memory.
  * Each request's film equals wl_pt_render of the same request bit for bit.
  * Requests may use different scenes of the server's device; their scenes must
  * have no guide attached, and their films must not overlap (rejected with
@@ -403,7 +414,8 @@ typedef struct wl_serve_config
     uint32_t batch_paths;          /* megabatch capacity in paths; must hold
                                       the largest request's pixel count     */
     uint32_t shade_blocks_per_cu;  /* shade kernel workgroups per CU        */
-    uint32_t use_graph;            /* replay a captured hipGraph            */
+    uint32_t use_graph;            /* launch the megabatch sequence as one
+                                      captured hipGraph (0: plain launches) */
     uint32_t lanes;                /* 2: requests split into two groups whose
                                       megabatches run concurrently (own
                                       buffers and stream each); 1: one      */
@@ -411,10 +423,14 @@ typedef struct wl_serve_config
 
 typedef struct wl_serve_stats
 {
-    double seconds;          /* GPU time from the first megabatch to the last */
+    double seconds;          /* device-clock time from the call's start (after
+                                work already on the stream) to the last
+                                megabatch's end */
     uint64_t paths;
     uint64_t extension_rays, shadow_rays;
-    uint32_t batches, pad_;
+    uint32_t batches;        /* megabatches over both lanes                */
+    uint32_t launches;       /* graph launches over both lanes: one per lane
+                                for up to 64 megabatches                    */
 } wl_serve_stats;
 
 wl_status wl_serve_default_config(wl_serve_config *cfg);
@@ -424,7 +440,9 @@ wl_status wl_server_get_config(wl_server server, wl_serve_config *cfg);
 wl_status wl_server_set_config(wl_server server, const wl_serve_config *cfg);
 /* Renders n requests; waits for work already on `stream` and returns when
  * every film is complete.  done_seconds (optional, n doubles): each request's
- * completion time from the start of the first megabatch. */
+ * completion time (the end of its last megabatch) from the call's start, on
+ * the device clock.  A lane's megabatches run as one captured graph launch
+ * (up to 64 megabatches per graph). */
 wl_status wl_server_render(wl_server server, const wl_render_request *reqs, uint32_t n,
                            wl_serve_stats *stats, double *done_seconds, void *stream);
 /* Times candidate megabatch sizes, shade grids and lane counts on the

@@ -46,6 +46,29 @@ FIXTURES = {
 }
 
 
+def load(name: str):
+    """(mean, var, meta) of a fixture: mean and var as float64 [h][w][3];
+    None if the fixture files are absent."""
+    base = os.path.join(HERE, name)
+    if not (os.path.exists(base + ".f32") and os.path.exists(base + ".json")):
+        return None
+    with open(base + ".json") as f:
+        meta = json.load(f)
+    blob = open(base + ".f32", "rb").read()
+    if hashlib.sha256(blob).hexdigest() != meta["sha256"]:
+        raise ValueError(f"{name}.f32 does not match its recorded SHA-256")
+    _, _, ww, wh = meta["window"]
+    a = np.frombuffer(blob, dtype="<f4").astype(np.float64).reshape(2, wh, ww, 3)
+    return a[0], a[1], meta
+
+
+def expected_relmse(mean, var, spp: int, ref_spp: int = SPP) -> float:
+    """E[relMSE] of an unbiased spp-sample estimate scored against the
+    fixture: mean over pixel-channels of (var/spp + var/ref_spp)/(R^2 + 0.01)
+    (the estimate's variance plus the fixture's own)."""
+    return float(np.mean((var / spp + var / ref_spp) / (mean ** 2 + 0.01)))
+
+
 def make(name: str, nthreads: int | None = None) -> dict:
     import oracle_bind as ob
     from wayland_amd import scenes

@@ -29,6 +29,8 @@ def lib():
         L.wlh_trace.restype = C.c_uint64
         L.wlh_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int,
                                 C.c_void_p, C.c_int]
+        L.wlh_serve_check.restype = C.c_uint64
+        L.wlh_serve_check.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
         _lib = L
     return _lib
 
@@ -62,3 +64,14 @@ class HostBVH8:
             lib().wlh_free(self.h)
         except Exception:
             pass
+
+
+def serve_check(reqs, capacity):
+    """Serving schedule of reqs ((width, height, spp, frame_offset) each) at
+    `capacity` paths per megabatch, walked with the multi-request kernels'
+    index arithmetic on the CPU: (violations, megabatches, items, paths,
+    largest megabatch)."""
+    a = np.ascontiguousarray(np.asarray(reqs, np.uint32).reshape(-1, 4))
+    out = (C.c_uint64 * 4)()
+    bad = lib().wlh_serve_check(a.ctypes.data, a.shape[0], capacity, out)
+    return (int(bad),) + tuple(int(x) for x in out)

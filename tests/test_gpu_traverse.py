@@ -117,3 +117,30 @@ def test_2m_scene_subsample_bit_exact(gpu_ctx):
     assert (h["t"][hit] >= 0).all() and np.isinf(h["t"][~hit]).all()
     del rays
     torch.cuda.empty_cache()
+
+
+def test_64x7_scene_subsample_bit_exact(gpu_ctx):
+    """The other point of the traversal workload: 64 icospheres at
+    subdivision 7 (20,971,520 triangles), 2^23 incoherent rays; every 256th
+    ray bit for bit against the oracle's BVH2, the rest by size-independent
+    properties."""
+    import torch
+    import wayland_amd as wl
+    print("case: 64 icospheres subdiv 7 (20,971,520 tris), 8388608 rays", flush=True)
+    sc = scenes.icosphere_scene(64, 7)
+    assert sc.num_triangles == 20_971_520
+    accel = wl.Accel(gpu_ctx, sc)
+    print(f"accel: {accel.info()}", flush=True)
+    rays = scenes.traversal_rays(1 << 23, seed=2)
+    h, occ = _gpu_hits(accel, rays)
+    sub = slice(0, None, 256)
+    obvh = ob.OracleBVH(sc.triangles())
+    _assert_bit_exact(h[sub], obvh.closest(rays[sub]))
+    assert np.array_equal(occ[sub], obvh.any(rays[sub]))
+    assert np.array_equal(occ.astype(bool), h["prim"] != 0xFFFFFFFF)
+    hit = h["prim"] != 0xFFFFFFFF
+    assert (h["prim"][hit] < sc.num_triangles).all()
+    assert (h["t"][hit] >= 0).all() and np.isinf(h["t"][~hit]).all()
+    del rays, obvh
+    accel.close()
+    torch.cuda.empty_cache()

@@ -559,6 +559,18 @@ class PathTracer:
                   _stream_ptr(film.device))
         return film
 
+    def trace_samples(self, cfg, frame, max_vertices=32):
+        """Per-sample radiance of one frame plus each path's per-vertex record
+        (H, W, max_vertices, 8): depth, mesh, pdf, mean f|cos|/pdf, mean
+        throughput reaching the vertex, 0, 0, trans (zeros past its end)."""
+        torch = _torch()
+        out = torch.empty((cfg.height, cfg.width, 3), dtype=torch.float32, device="cuda")
+        tr = torch.zeros((cfg.height, cfg.width, max_vertices, 8), dtype=torch.float32,
+                         device="cuda")
+        _lib.call("wl_pt_trace_samples", self.h, C.byref(cfg), frame, C.c_void_p(out.data_ptr()),
+                  C.c_void_p(tr.data_ptr()), max_vertices, _stream_ptr(out.device))
+        return out, tr
+
     def render_samples(self, cfg, frame, out=None):
         torch = _torch()
         if out is None:

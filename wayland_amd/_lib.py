@@ -102,7 +102,7 @@ class ServeConfig(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("paths", C.c_uint64),
                 ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
-                ("batches", C.c_uint32), ("pad_", C.c_uint32)]
+                ("batches", C.c_uint32), ("launches", C.c_uint32)]
 
 
 class ProgramGroup(C.Structure):
@@ -182,6 +182,8 @@ SIGNATURES = {
                                         C.POINTER(MeshTexture), _U32]),
     "wl_pt_render": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, C.POINTER(PTStats), _VP]),
     "wl_pt_render_noalloc": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, _VP]),
+    "wl_pt_trace_samples": (C.c_int, [_VP, C.POINTER(PTConfig), C.c_uint32, _VP, _VP,
+                                      C.c_uint32, _VP]),
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),
     "wl_pt_launch_frame": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP, _VP]),
     "wl_film_resolve": (C.c_int, [_VP, _VP, _U32, _U32, _VP, _VP]),

@@ -1,8 +1,11 @@
-// hostemu.cpp — CPU emulation of the BVH8 builder + traversal kernels, for
-// tests only (libwl_hostemu.so).  It compiles the exact traversal code of
-// traverse_device.h for the host, with every node index, triangle index and
-// stack slot the kernel would touch checked against its allocation, so the
-// indexing is verified on the CPU before any kernel runs on a GPU.
+// hostemu.cpp — CPU emulation of the BVH8 builder + traversal kernels and of
+// the batched-serving index arithmetic, for tests only (libwl_hostemu.so).
+// It compiles the exact traversal code of traverse_device.h for the host,
+// with every node index, triangle index and stack slot the kernel would touch
+// checked against its allocation, and the serving schedule (serve_plan.h:
+// plan, find_item) with every path, pixel and film index the multi-request
+// raygen and film-update kernels derive from it checked, so the indexing is
+// verified on the CPU before any kernel runs on a GPU.
 #include <atomic>
 #include <cstring>
 #include <string>
@@ -10,6 +13,7 @@
 #include <vector>
 
 #include "bvh_builder.h"
+#include "serve_plan.h"
 
 static thread_local size_t wl_trav_nodes_bound = 0;
 static thread_local size_t wl_trav_tris_bound = 0;
@@ -132,6 +136,128 @@ uint64_t wlh_trace(void *p, const float *rays, uint32_t n, float *hits,
     for (auto &x : th)
         x.join();
     return g_faults.load();
+}
+
+// Serving schedule of n requests (req4: width, height, spp, frame_offset
+// each) in megabatches of `capacity` paths, walked as the multi-request
+// kernels walk it: every path of every megabatch through find_item to its
+// item, pixel and sample (k_raygen_multi), every pixel of the megabatch's
+// pixel list to its item and samples (k_accumulate_multi).  Counts each
+// violation: an index outside the batch buffers, the item or the request's
+// film; a path or pixel mapped to an item that does not contain it; a
+// request sample (pixel, frame) covered other than exactly once, or a
+// pixel's samples reaching its film out of ascending frame order.
+// out4: megabatches, items, paths, largest megabatch.
+uint64_t wlh_serve_check(const uint32_t *req4, uint32_t n, uint32_t capacity, uint64_t *out4)
+{
+    std::vector<wl::PlanRequest> r(n);
+    for (uint32_t k = 0; k < n; ++k)
+        r[k] = { req4[4 * k], req4[4 * k + 1], req4[4 * k + 2], req4[4 * k + 3] };
+    wl::Plan P;
+    wl::plan(r.data(), n, capacity, P);
+    uint64_t bad = 0, maxp = 0;
+    // per request: samples seen per pixel (counts), last frame added per pixel
+    std::vector<std::vector<uint32_t>> seen(n), last(n);
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        seen[k].assign((size_t)r[k].width * r[k].height, 0);
+        last[k].assign((size_t)r[k].width * r[k].height, 0xFFFFFFFFu);
+    }
+    std::vector<uint32_t> items_of_req(n);
+    for (size_t b = 0; b < P.batches.size(); ++b)
+    {
+        const wl::PtBatchHeader &h = P.batches[b];
+        maxp = std::max<uint64_t>(maxp, h.paths);
+        if (h.paths > capacity || h.pixels > capacity ||
+            (uint64_t)h.item_offset + h.items > P.items.size())
+        {
+            ++bad;
+            continue;
+        }
+        const wl::PtItem *items = P.items.data() + h.item_offset;
+        std::fill(items_of_req.begin(), items_of_req.end(), 0u);
+        for (uint32_t i = 0; i < h.items; ++i)
+        {
+            if (items[i].req >= n)
+                ++bad;
+            else if (++items_of_req[items[i].req] > 1) // one item per request
+                ++bad;
+        }
+        // k_raygen_multi: path p -> item, pixel, sample
+        for (uint32_t p = 0; p < h.paths; ++p)
+        {
+            const uint32_t j = wl::find_item(items, h.items, p, false);
+            if (j >= h.items)
+            {
+                ++bad;
+                continue;
+            }
+            const wl::PtItem &it = items[j];
+            if (p < it.path0 || p >= it.path0 + it.npix * it.nsamples)
+            {
+                ++bad;
+                continue;
+            }
+            const uint32_t local = p - it.path0, pixel = local % it.npix, s = local / it.npix;
+            const wl::PlanRequest &q = r[it.req];
+            if (pixel >= q.width * q.height || pixel % q.width >= q.width ||
+                pixel / q.width >= q.height || s >= it.nsamples ||
+                it.frame0 + s >= q.frame_offset + q.spp || it.frame0 + s < q.frame_offset)
+                ++bad;
+        }
+        // k_accumulate_multi: megabatch pixel x -> item, film pixel, its samples
+        for (uint32_t x = 0; x < h.pixels; ++x)
+        {
+            const uint32_t j = wl::find_item(items, h.items, x, true);
+            if (j >= h.items)
+            {
+                ++bad;
+                continue;
+            }
+            const wl::PtItem &it = items[j];
+            if (x < it.px0 || x >= it.px0 + it.npix)
+            {
+                ++bad;
+                continue;
+            }
+            const uint32_t px = x - it.px0;
+            const wl::PlanRequest &q = r[it.req];
+            if (px >= (uint64_t)q.width * q.height)
+            {
+                ++bad;
+                continue;
+            }
+            for (uint32_t s = 0; s < it.nsamples; ++s)
+            {
+                const uint64_t slot = it.path0 + (uint64_t)s * it.npix + px;
+                if (slot >= h.paths || slot >= capacity)
+                    ++bad;
+                const uint32_t f = it.frame0 + s;
+                uint32_t &l = last[it.req][px];
+                if (l != 0xFFFFFFFFu && f <= l)
+                    ++bad; // not ascending
+                l = f;
+                ++seen[it.req][px];
+            }
+        }
+    }
+    for (uint32_t k = 0; k < n; ++k)
+    {
+        for (size_t px = 0; px < seen[k].size(); ++px)
+            if (seen[k][px] != r[k].spp ||
+                (r[k].spp && last[k][px] != r[k].frame_offset + r[k].spp - 1))
+                ++bad;
+        if (r[k].spp && P.last_batch[k] >= P.batches.size())
+            ++bad;
+    }
+    if (out4)
+    {
+        out4[0] = P.batches.size();
+        out4[1] = P.items.size();
+        out4[2] = P.paths;
+        out4[3] = maxp;
+    }
+    return bad;
 }
 
 } // extern "C"

@@ -26,6 +26,7 @@
 #include "guide_device.h"
 #include "shading_device.h"
 #include "traverse_persistent.h"
+#include "serve_plan.h"
 #include "wl_pt.h"
 
 namespace wl
@@ -375,7 +376,8 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 // and parameters.
 // GTRAIN: paths carry guide training records (radiance after the recorded
 // vertex is summed; GSAMPLE vertices record themselves).
-template <bool GSAMPLE, bool GTRAIN, bool MULTI>
+// TRACE: also writes the per-vertex record of PtBuffers::trace.
+template <bool GSAMPLE, bool GTRAIN, bool MULTI, bool TRACE = false>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(3)))
 void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__restrict__ reqs,
              int first)
@@ -586,6 +588,17 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                         V3 bsdf = eval_disney(mat, Ns, Ng, -rayDir, nd, tex);
                         bsdf = clamp3(bsdf, 0.f, 1e30f);
                         lastTraceTerm = bsdf * fabsf(cosWeight) / pdf;
+                        if (TRACE && depth - 1 < B.trace_max)
+                        {
+                            float4 *tr = reinterpret_cast<float4 *>(
+                                B.trace + 8 * ((size_t)path * B.trace_max + depth - 1));
+                            tr[0] = make_float4((float)depth, (float)mesh, pdf,
+                                                (lastTraceTerm.x + lastTraceTerm.y +
+                                                 lastTraceTerm.z) / 3.0f);
+                            tr[1] = make_float4((throughput.x + throughput.y + throughput.z) /
+                                                    3.0f,
+                                                0.0f, 0.0f, mat.trans);
+                        }
                         const V3 lastRayDir = rayDir;
                         if (GSAMPLE && guided && gtrain && !grec_done)
                         {
@@ -777,23 +790,6 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
 }
 
 // ---- batched serving: raygen and film update of a multi-request megabatch ----
-// Item of megabatch entry x given the items' start offsets (binary search).
-__device__ __forceinline__ uint32_t find_item(const PtItem *items, uint32_t nitems, uint32_t x,
-                                              bool by_pixel)
-{
-    uint32_t lo = 0, hi = nitems; // items[lo].start <= x < items[hi].start
-    while (hi - lo > 1)
-    {
-        const uint32_t mid = (lo + hi) / 2;
-        const uint32_t st = by_pixel ? items[mid].px0 : items[mid].path0;
-        if (st <= x)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 __global__ void k_batch_begin(PtMultiLaunch L)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -803,7 +799,17 @@ __global__ void k_batch_begin(PtMultiLaunch L)
 __global__ void k_batch_end(PtMultiLaunch L)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0)
-        *L.batch_index += 1;
+    {
+        const uint32_t b = *L.batch_index;
+        L.times[b] = (unsigned long long)wall_clock64(); // the batch's film update is done
+        *L.batch_index = b + 1;
+    }
+}
+
+__global__ void k_clock_stamp(unsigned long long *dst)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *dst = (unsigned long long)wall_clock64();
 }
 
 __global__ __launch_bounds__(kPtBlock) void k_raygen_multi(PtMultiLaunch L, PtBuffers B)
@@ -1060,6 +1066,9 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
         else if (gtrain)
             hipLaunchKernelGGL((k_shade<false, true, false>), dim3(g_shade), dim3(kPtBlock), 0, st,
                                P, S, B, parity, nullptr, b == 0 ? 1 : 0);
+        else if (B.trace)
+            hipLaunchKernelGGL((k_shade<false, false, false, true>), dim3(g_shade),
+                               dim3(kPtBlock), 0, st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
         else
             hipLaunchKernelGGL((k_shade<false, false, false>), dim3(g_shade), dim3(kPtBlock), 0,
                                st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
@@ -1089,6 +1098,12 @@ void pt_multi_init()
             occ32 < 1)
             occ32 = 4;
     }
+}
+
+hipError_t pt_clock_stamp(unsigned long long *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_clock_stamp, dim3(1), dim3(64), 0, st, dst);
+    return hipGetLastError();
 }
 
 hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus, hipStream_t st)

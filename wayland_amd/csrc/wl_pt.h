@@ -108,6 +108,12 @@ struct PtBuffers
     uint32_t *gcount; // number of training samples
     uint32_t gtrain_cap;
     float gwmax;      // training weights are clamped to this
+    // per-vertex record of each path (wl_pt_trace_samples only, else NULL):
+    // 8 floats per surface vertex that sampled a direction, at
+    // trace[(path * trace_max + depth - 1) * 8]: depth, mesh, pdf, mean
+    // f|cos|/pdf, mean throughput reaching the vertex, 0, 0, trans
+    float *trace;
+    uint32_t trace_max;
 };
 
 hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
@@ -153,6 +159,7 @@ struct PtMultiLaunch
     const PtBatchHeader *batches; // all batches' headers
     uint32_t *batch_index;        // device counter of the batch being run
     PtBatchHeader *header;        // the current batch's header
+    unsigned long long *times;    // device wall clock at each batch's end
     // the requests' BVHs back to back, child and triangle indices rebased, so
     // multi-request traversal keeps the single-BVH kernel's registers and
     // starts each ray at its request's root
@@ -170,6 +177,8 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
                         hipStream_t st);
 // One-time occupancy queries of the multi-request kernels (outside capture).
 void pt_multi_init();
+// Writes the device wall clock (wall_clock64) to *dst once the stream gets there.
+hipError_t pt_clock_stamp(unsigned long long *dst, hipStream_t st);
 // Copies n BVH nodes into an arena at node offset node0, adding node0 to the
 // child indices and tri0 to the triangle indices (the triangle records go to
 // tri0 unchanged).

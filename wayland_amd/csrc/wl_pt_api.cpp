@@ -708,6 +708,41 @@ wl_status wl_pt_render_samples(wl_scene s, const wl_pt_config *c, uint32_t frame
     return WL_SUCCESS;
 }
 
+wl_status wl_pt_trace_samples(wl_scene s, const wl_pt_config *c, uint32_t frame, float *rgb,
+                              float *trace, uint32_t max_vertices, void *stream)
+{
+    wl_status st = check_config(s, c);
+    if (st != WL_SUCCESS)
+        return st;
+    if (!rgb || !trace)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL buffer");
+    if (max_vertices == 0 || max_vertices > 1024)
+        return set_error(WL_ERROR_INVALID_VALUE, "max_vertices must be in [1, 1024]");
+    if (s->guide && s->guide_mode)
+        return set_error(WL_ERROR_NOT_SUPPORTED, "traced renders are unguided");
+    const uint32_t npix = c->width * c->height;
+    const uint32_t max_paths = c->max_batch_paths ? c->max_batch_paths : kDefaultBatchPaths;
+    if (npix > max_paths)
+        return set_error(WL_ERROR_LIMIT_EXCEEDED, "a traced render is one batch");
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    st = ensure_capacity(s, npix);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtBuffers B = carve(s);
+    B.trace = trace;
+    B.trace_max = max_vertices;
+    st = one_frame(s, c, frame, B, hs, 0, npix);
+    if (st != WL_SUCCESS)
+        return st;
+    wl::PtParams P = params_for(s, c);
+    P.pix0 = 0;
+    P.npix = npix;
+    P.nsamples = 1;
+    WL_HIP_CHECK(wl::pt_store_samples(P, B, rgb, hs));
+    return WL_SUCCESS;
+}
+
 wl_status wl_pt_launch_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
                              float *radiance_rgba, uint8_t *color_rgba, void *stream)
 {

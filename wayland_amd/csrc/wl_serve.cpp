@@ -3,11 +3,12 @@
 // megabatches.  The reference renders one launch per request and frame
 // (DeviceManager::Launch, src/Frontend/Core/DeviceManager.cpp:231-241);
 // here the scheduler packs whole requests x sample ranges into megabatches
-// of up to batch_paths paths, each megabatch's kernels (raygen, max_depth x
-// extend / shade / shadow, film update) are captured once into a hipGraph,
-// and the graph is replayed once per megabatch: the batch descriptors live in
-// device memory and the graph's first kernel picks the next one, so the host
-// enqueues nothing but graph launches.  With two lanes the requests are
+// of up to batch_paths paths, and a render call's whole megabatch sequence
+// (per megabatch: raygen, max_depth x extend / shade / shadow, film update)
+// is captured into one hipGraph and launched once: the batch descriptors live
+// in device memory and each megabatch's first kernel picks the next one, so
+// the host enqueues one graph launch per lane (one per kGraphBatches
+// megabatches beyond that).  With two lanes the requests are
 // split into two groups (each request whole in one group) whose megabatch
 // sequences run concurrently on two streams with their own buffers, so each
 // lane's kernels fill the other's tails.  An autotuner times candidate
@@ -17,6 +18,7 @@
 // a path's arithmetic depends only on its own state and its request's scene,
 // and every pixel receives its samples in ascending order.
 #include "wl_internal.h"
+#include "serve_plan.h"
 #include "wl_pt.h"
 
 #include <algorithm>
@@ -51,10 +53,16 @@ struct wl_server_s
     uint32_t arena_nodes_cap = 0;
     WlTriRecord *d_arena_tris = nullptr;
     uint32_t arena_tris_cap = 0;
-    // captured megabatch and everything it bakes in (compared field by field)
-    hipGraphExec_t exec = nullptr;
-    std::vector<uint64_t> graph_key;
-    std::vector<hipEvent_t> events;
+    // captured megabatch sequences and everything they bake in (compared field
+    // by field): `exec` runs graph_batches megabatches, `exec_rest` the
+    // remainder of a call with more than kGraphBatches
+    hipGraphExec_t exec = nullptr, exec_rest = nullptr;
+    std::vector<uint64_t> graph_key, rest_key;
+    // device wall clock at the end of each megabatch, and at the call's start
+    unsigned long long *d_times = nullptr;
+    uint32_t times_cap = 0;
+    unsigned long long *d_clock = nullptr;
+    double clock_hz = 1.0e8;
     // second lane (created on first use): with cfg.lanes = 2 the requests are
     // split into two groups whose megabatches run concurrently, the second
     // group through this server's twin (own buffers, stream and graph)
@@ -65,6 +73,8 @@ namespace
 {
 
 constexpr uint32_t kDefaultServeBatch = 1u << 22;
+// megabatches per captured graph (about 50 kernel nodes each at depth 16)
+constexpr uint32_t kGraphBatches = 64;
 
 template <class T>
 wl_status grow(T **p, uint32_t &cap, uint32_t need)
@@ -91,7 +101,51 @@ void drop_graph(wl_server s)
         hipGraphExecDestroy(s->exec);
         s->exec = nullptr;
     }
+    if (s->exec_rest)
+    {
+        hipGraphExecDestroy(s->exec_rest);
+        s->exec_rest = nullptr;
+    }
     s->graph_key.clear();
+    s->rest_key.clear();
+}
+
+// Captures `count` consecutive megabatches into *exec (replacing it) unless
+// it already holds them for `key`.
+wl_status capture_batches(wl_server s, const wl::PtMultiLaunch &L, const wl::PtBuffers &B,
+                          uint32_t count, const std::vector<uint64_t> &key,
+                          hipGraphExec_t *exec, std::vector<uint64_t> *held)
+{
+    if (*exec && *held == key)
+        return WL_SUCCESS;
+    if (*exec)
+    {
+        hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+        held->clear();
+    }
+    wl::pt_multi_init();
+    hipGraph_t g = nullptr;
+    WL_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = hipSuccess;
+    for (uint32_t b = 0; b < count && e1 == hipSuccess; ++b)
+        e1 = wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream);
+    const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess)
+    {
+        if (g)
+            hipGraphDestroy(g);
+        return wl::hip_error(e1 != hipSuccess ? e1 : e2, "megabatch capture");
+    }
+    const hipError_t e3 = hipGraphInstantiateWithFlags(exec, g, 0);
+    hipGraphDestroy(g);
+    if (e3 != hipSuccess)
+    {
+        *exec = nullptr;
+        return wl::hip_error(e3, "megabatch graph instantiate");
+    }
+    *held = key;
+    return WL_SUCCESS;
 }
 
 wl_status ensure_buffers(wl_server s, uint32_t capacity)
@@ -143,68 +197,12 @@ wl_status normalise_config(wl_serve_config &c)
     return WL_SUCCESS;
 }
 
-struct Plan
-{
-    std::vector<wl::PtItem> items;
-    std::vector<wl::PtBatchHeader> batches;
-    std::vector<uint32_t> last_batch; // per request
-    uint64_t paths = 0;
-};
-
-// FIFO packing: each megabatch takes, in request order, as many whole-image
-// sample passes of every unfinished request as still fit.  (Items stay in
-// request order: grouping a megabatch's items by BVH was measured slower,
-// since the traversal's static per-wave ray ranges then get uneven work.)
-void plan(const wl_render_request *r, uint32_t n, uint32_t capacity, Plan &P)
-{
-    std::vector<uint32_t> rem(n), next(n, 0);
-    P.last_batch.assign(n, 0);
-    uint32_t pending = 0;
-    for (uint32_t k = 0; k < n; ++k)
-    {
-        rem[k] = r[k].spp;
-        pending += rem[k] > 0;
-    }
-    while (pending)
-    {
-        wl::PtBatchHeader h{ 0, 0, 0, (uint32_t)P.items.size() };
-        uint32_t free_paths = capacity;
-        for (uint32_t k = 0; k < n && free_paths; ++k)
-        {
-            if (!rem[k])
-                continue;
-            const uint32_t npix = r[k].width * r[k].height;
-            if (npix > free_paths)
-                continue;
-            const uint32_t ns = std::min(rem[k], free_paths / npix);
-            wl::PtItem it{};
-            it.req = k;
-            it.npix = npix;
-            it.frame0 = r[k].frame_offset + next[k];
-            it.nsamples = ns;
-            it.path0 = h.paths;
-            it.px0 = h.pixels;
-            P.items.push_back(it);
-            h.paths += ns * npix;
-            h.pixels += npix;
-            h.items += 1;
-            free_paths -= ns * npix;
-            next[k] += ns;
-            rem[k] -= ns;
-            P.last_batch[k] = (uint32_t)P.batches.size();
-            if (!rem[k])
-                --pending;
-        }
-        P.paths += h.paths;
-        P.batches.push_back(h);
-    }
-}
-
 // What a lane enqueued, for finish_lane.
 struct LaneRun
 {
-    Plan pl;
+    wl::Plan pl;
     uint32_t nb = 0;
+    uint32_t launches = 0; // graph launches (or megabatch sequences) enqueued
 };
 
 // Checks every request before any lane enqueues work, so an invalid request
@@ -302,8 +300,11 @@ wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_requ
     if (max_npix > cfg.batch_paths)
         return set_error(WL_ERROR_LIMIT_EXCEEDED,
                          "batch_paths must hold the largest request's pixel count");
-    Plan &pl = out.pl;
-    plan(reqs, n, cfg.batch_paths, pl);
+    wl::Plan &pl = out.pl;
+    std::vector<wl::PlanRequest> preq(n);
+    for (uint32_t k = 0; k < n; ++k)
+        preq[k] = { reqs[k].width, reqs[k].height, reqs[k].spp, reqs[k].frame_offset };
+    wl::plan(preq.data(), n, cfg.batch_paths, pl);
     const uint32_t nb = (uint32_t)pl.batches.size();
     out.nb = nb;
     st = ensure_buffers(s, cfg.batch_paths);
@@ -313,6 +314,8 @@ wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_requ
         st = grow(&s->d_items, s->items_cap, (uint32_t)pl.items.size());
     if (st == WL_SUCCESS)
         st = grow(&s->d_batches, s->batches_cap, nb);
+    if (st == WL_SUCCESS)
+        st = grow(&s->d_times, s->times_cap, std::max<uint32_t>(nb, 1));
     if (st == WL_SUCCESS)
         st = grow(&s->d_arena_nodes, s->arena_nodes_cap, (uint32_t)arena_nodes);
     if (st == WL_SUCCESS)
@@ -333,6 +336,7 @@ wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_requ
     L.batches = s->d_batches;
     L.batch_index = s->d_batch_index;
     L.header = s->d_header;
+    L.times = s->d_times;
     L.nodes = s->d_arena_nodes;
     L.tris = s->d_arena_tris;
     L.max_depth = max_depth;
@@ -356,10 +360,13 @@ wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_requ
                                         hipMemcpyDeviceToDevice, s->stream));
     }
 
-    if (cfg.use_graph)
+    // the graphs bake in buffer pointers, grids, the bounce count and the
+    // number of megabatches they run
+    const uint32_t per_graph = std::min(nb, kGraphBatches);
+    const uint32_t full = per_graph ? nb / per_graph : 0, rest = per_graph ? nb % per_graph : 0;
+    if (cfg.use_graph && nb)
     {
-        // the graph bakes in buffer pointers, grids and the bounce count
-        const std::vector<uint64_t> key = {
+        std::vector<uint64_t> key = {
             max_depth,
             L.stack16,
             cfg.shade_blocks_per_cu,
@@ -370,71 +377,54 @@ wl_status run_lane(wl_server s, const wl_serve_config &cfg, const wl_render_requ
             (uint64_t)(uintptr_t)s->d_batches,
             (uint64_t)(uintptr_t)s->d_arena_nodes,
             (uint64_t)(uintptr_t)s->d_arena_tris,
+            (uint64_t)(uintptr_t)s->d_times,
+            per_graph,
         };
-        if (!s->exec || s->graph_key != key)
+        st = capture_batches(s, L, B, per_graph, key, &s->exec, &s->graph_key);
+        if (st == WL_SUCCESS && rest)
         {
-            drop_graph(s);
-            wl::pt_multi_init();
-            hipGraph_t g = nullptr;
-            WL_HIP_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            const hipError_t e1 = wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream);
-            const hipError_t e2 = hipStreamEndCapture(s->stream, &g);
-            if (e1 != hipSuccess || e2 != hipSuccess)
-            {
-                if (g)
-                    hipGraphDestroy(g);
-                return wl::hip_error(e1 != hipSuccess ? e1 : e2, "megabatch capture");
-            }
-            const hipError_t e3 = hipGraphInstantiateWithFlags(&s->exec, g, 0);
-            hipGraphDestroy(g);
-            if (e3 != hipSuccess)
-            {
-                s->exec = nullptr;
-                return wl::hip_error(e3, "megabatch graph instantiate");
-            }
-            s->graph_key = key;
+            key.back() = rest;
+            st = capture_batches(s, L, B, rest, key, &s->exec_rest, &s->rest_key);
         }
+        if (st != WL_SUCCESS)
+            return st;
     }
-    while (s->events.size() < (size_t)nb + 1)
-    {
-        hipEvent_t e;
-        WL_HIP_CHECK(hipEventCreate(&e));
-        s->events.push_back(e);
-    }
-    // counters: zeroed by a plain copy before the replays (never inside the graph)
+    // counters: zeroed by a plain copy before the launch (never inside the graph)
     const uint32_t zeros[16] = {};
     WL_HIP_CHECK(hipMemcpyAsync(s->d_batch_index, zeros, 4, hipMemcpyHostToDevice, s->stream));
     WL_HIP_CHECK(hipMemcpyAsync(s->d_stats, zeros, 16, hipMemcpyHostToDevice, s->stream));
-    WL_HIP_CHECK(hipEventRecord(s->events[0], s->stream));
-    for (uint32_t b = 0; b < nb; ++b)
+    if (cfg.use_graph)
     {
-        if (cfg.use_graph)
+        for (uint32_t k = 0; k < full; ++k)
             WL_HIP_CHECK(hipGraphLaunch(s->exec, s->stream));
-        else
-            WL_HIP_CHECK(wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream));
-        WL_HIP_CHECK(hipEventRecord(s->events[b + 1], s->stream));
+        if (rest)
+            WL_HIP_CHECK(hipGraphLaunch(s->exec_rest, s->stream));
     }
+    else
+    {
+        for (uint32_t b = 0; b < nb; ++b)
+            WL_HIP_CHECK(wl::pt_run_multi(L, B, s->ctx->num_cus, s->stream));
+    }
+    out.launches = cfg.use_graph ? full + (rest ? 1u : 0u) : nb;
     return WL_SUCCESS;
 }
 
-// Waits for a lane; its megabatch end times relative to `origin` (an event
-// recorded before any lane started), per request in `done` (lane order), and
-// its ray counts added to `rays`.
+// Waits for a lane; its megabatch end times relative to `origin` (the device
+// clock stamped on the caller's stream before any lane started), per request
+// in `done` (lane order), and its ray counts added to `rays`.
 wl_status finish_lane(wl_server s, const LaneRun &lr, const wl_render_request *reqs, uint32_t n,
-                      hipEvent_t origin, double *done, double &end,
+                      unsigned long long origin, double hz, double *done, double &end,
                       unsigned long long rays[2])
 {
     WL_HIP_CHECK(hipStreamSynchronize(s->stream));
-    std::vector<double> tb(lr.nb + 1, 0.0);
-    for (uint32_t b = 0; b <= lr.nb; ++b)
-    {
-        float ms = 0.f;
-        WL_HIP_CHECK(hipEventElapsedTime(&ms, origin, s->events[b]));
-        tb[b] = ms * 1e-3;
-    }
-    end = tb[lr.nb];
+    std::vector<unsigned long long> tb(lr.nb);
+    if (lr.nb)
+        WL_HIP_CHECK(hipMemcpy(tb.data(), s->d_times, sizeof(unsigned long long) * lr.nb,
+                               hipMemcpyDeviceToHost));
+    auto secs = [&](unsigned long long t) { return t > origin ? (double)(t - origin) / hz : 0.0; };
+    end = lr.nb ? secs(tb[lr.nb - 1]) : 0.0;
     for (uint32_t k = 0; k < n; ++k)
-        done[k] = reqs[k].spp ? tb[lr.pl.last_batch[k] + 1] : 0.0;
+        done[k] = reqs[k].spp ? secs(tb[lr.pl.last_batch[k]]) : 0.0;
     unsigned long long h[2] = { 0, 0 };
     WL_HIP_CHECK(hipMemcpy(h, s->d_stats, 16, hipMemcpyDeviceToHost));
     rays[0] += h[0];
@@ -486,15 +476,19 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
         for (uint32_t k : idx[l])
             sub[l].push_back(reqs[k]);
     wl_server lane[2] = { s, s->twin };
-    hipEvent_t origin;
-    WL_HIP_CHECK(hipEventCreate(&origin));
+    // the call's start on the device clock, once the work already queued on
+    // `user` is done (each lane starts after it)
+    WL_HIP_CHECK(wl::pt_clock_stamp(s->d_clock, user));
     LaneRun lr[2];
     st = WL_SUCCESS;
     for (uint32_t l = 0; l < used && st == WL_SUCCESS; ++l)
-    {
-        if (l == 0)
-            WL_HIP_CHECK(hipEventRecord(origin, user));
         st = run_lane(lane[l], cfg, sub[l].data(), (uint32_t)sub[l].size(), user, lr[l]);
+    unsigned long long origin = 0;
+    if (st == WL_SUCCESS)
+    {
+        // the lanes waited for the stamp: it is written once a lane is done
+        WL_HIP_CHECK(hipStreamSynchronize(s->stream));
+        WL_HIP_CHECK(hipMemcpy(&origin, s->d_clock, sizeof origin, hipMemcpyDeviceToHost));
     }
     std::vector<double> done[2];
     double end = 0.0;
@@ -504,7 +498,7 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
         done[l].assign(sub[l].size(), 0.0);
         double e = 0.0;
         st = finish_lane(lane[l], lr[l], sub[l].data(), (uint32_t)sub[l].size(), origin,
-                         done[l].data(), e, rays);
+                         s->clock_hz, done[l].data(), e, rays);
         end = std::max(end, e);
     }
     if (st != WL_SUCCESS)
@@ -514,7 +508,6 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
             if (lane[l])
                 hipStreamSynchronize(lane[l]->stream);
     }
-    hipEventDestroy(origin);
     if (st != WL_SUCCESS)
         return st;
     if (done_seconds)
@@ -524,13 +517,14 @@ wl_status run(wl_server s, const wl_render_request *reqs, uint32_t n, wl_serve_s
     if (stats)
     {
         std::memset(stats, 0, sizeof *stats);
-        // from the call's start (work already queued on `user` included)
-        // to the end of the last lane
+        // from the call's start (after the work already queued on `user`)
+        // to the end of the last lane, on the device clock
         stats->seconds = end;
         for (uint32_t l = 0; l < used; ++l)
         {
             stats->paths += lr[l].pl.paths;
             stats->batches += lr[l].nb;
+            stats->launches += lr[l].launches;
         }
         stats->extension_rays = rays[0];
         stats->shadow_rays = rays[1];
@@ -555,6 +549,13 @@ wl_status create_server(wl_context ctx, const wl_serve_config &c, wl_server *out
         e = hipMalloc((void **)&s->d_batch_index, 64);
     if (e == hipSuccess)
         e = hipMalloc((void **)&s->d_header, 64);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_clock, 64);
+    int khz = 0;
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
+    if (e == hipSuccess && khz > 0)
+        s->clock_hz = 1e3 * khz;
     if (e != hipSuccess)
     {
         destroy_server(s);
@@ -576,9 +577,9 @@ wl_status destroy_server(wl_server s)
     hipSetDevice(s->ctx->device);
     hipDeviceSynchronize();
     drop_graph(s);
-    for (hipEvent_t e : s->events)
-        hipEventDestroy(e);
     hipFree(s->d_buf);
+    hipFree(s->d_times);
+    hipFree(s->d_clock);
     hipFree(s->d_counters);
     hipFree(s->d_stats);
     hipFree(s->d_reqs);

@@ -448,3 +448,29 @@ def glossy_caustic(width=1024, height=1024, sphere_subdiv=6, floor_n=243) -> Sce
     cam = pinhole_camera([0.0, 0.8, 3.5], [0.0, -0.8, -2.0], [0.0, 1.0, 0.0], 35.0,
                          width, height)
     return Scene(meshes, cam, (0.0, 0.0, 0.0), "glossy-caustic")
+
+
+def serving_scenes():
+    """The scenes of the batched-serving workload: four Cornell-box variants
+    (base colours from seeds 0..3) and four icosphere variants (8 spheres at
+    subdivision 4, materials and lights of the wavefront workload, seeds
+    10..13), each with a square 256 x 256 camera."""
+    out = [cornell_box(256, 256, seed=k) for k in range(4)]
+    out += [wavefront_scene(8, 4, seed=10 + k, width=256, height=256) for k in range(4)]
+    return out
+
+
+def serving_requests(rng, n, res_lo, res_hi, spp_hi, seed_shift, num_scenes=8):
+    """n render requests of the serving workload drawn from rng: scene index
+    uniform over the serving scenes, square side uniform in [res_lo, res_hi],
+    spp uniform in [1, spp_hi], depth uniform in [8, 16], frame offset
+    (k + seed_shift) * 2^16 for request k.  The 64-request mix is
+    serving_requests(np.random.default_rng(5), 64, 128, 1024, 64, 0)."""
+    reqs = []
+    for k in range(n):
+        res = int(rng.integers(res_lo, res_hi + 1))
+        reqs.append({"scene": int(rng.integers(0, num_scenes)), "width": res, "height": res,
+                     "spp": int(rng.integers(1, spp_hi + 1)),
+                     "max_depth": int(rng.integers(8, 17)),
+                     "frame_offset": (k + seed_shift) << 16})
+    return reqs
