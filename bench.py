@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Mpaths/s + Grays/s; relMSE at equal time vs FP64 oracle, guided vs plain PT"
 HBM_PEAK_GBS = 8000.0
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 16, 128, 48
+PT_RAY_HIT_BYTES, PT_SHADE_BYTES, PT_SHADOW_BYTES = 48, 380, 52
 
 
 def parse():
@@ -140,12 +141,28 @@ def bench_pt(args, ctx, accel, sc, wl, torch):
     dt_sorted = start.elapsed_time(end) / 1e3
     pt.close()
     waccel.close()
+    # path-state and queue bytes of the render (BVH node and triangle
+    # fetches excluded): per extension ray the ray and hit (48 B) plus the
+    # shade step (380 B: hit, path id, state read and write, shading record,
+    # material, next ray, shadow ray and contribution); per shadow ray its ray,
+    # path id and contribution (52 B)
+    ext = st["extension_rays"] / st["paths"] * paths
+    shd = st["shadow_rays"] / st["paths"] * paths
+    state_bytes = ext * (PT_RAY_HIT_BYTES + PT_SHADE_BYTES) + shd * PT_SHADOW_BYTES
+    traffic = traffic_from_profiles(f"pt_wavefront_{W}x{H}", "traffic_bytes_per_path")
     out = {"workload": f"wavefront PT {W}x{H} {args.pt_spp} spp depth 16 RR>4, "
                        f"{wsc.num_triangles} tris, 2 quad lights + env 0.1",
            "mpaths_per_s": paths / dt / 1e6,
            "grays_per_s": paths * rays_per_path / dt / 1e9,
            "rays_per_path": rays_per_path, "seconds": dt,
-           "mpaths_per_s_material_sorted": paths / dt_sorted / 1e6}
+           "mpaths_per_s_material_sorted": paths / dt_sorted / 1e6,
+           "roofline": {"bound": "hbm", "achieved": state_bytes / dt / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": state_bytes / dt / 1e9 / HBM_PEAK_GBS,
+                        "algorithmic_bytes": state_bytes,
+                        "bytes": "path state + queues only (48 B ray/hit + 380 B shade per "
+                                 "extension ray, 52 B per shadow ray); BVH fetches excluded",
+                        "traffic_per_path": traffic}}
     log(f"pt: {out}")
     return out
 
@@ -488,13 +505,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(tag):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, copy-calibrated), recorded in profiles/<tag>_pmc.md."""
+def traffic_from_profiles(tag, key="traffic_bytes_per_launch"):
+    """HBM bytes measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), recorded as `key: value` in profiles/<tag>_pmc.md."""
     path = os.path.join(ROOT, "profiles", f"{tag}_pmc.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(key + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 

@@ -383,6 +383,16 @@ wl_status wl_guide_gradient(wl_guide guide, const float *d_samples, uint32_t n,
 /* `steps` Adam steps over consecutive mini-batches of the n samples. */
 wl_status wl_guide_train(wl_guide guide, const float *d_samples, uint32_t n,
                          uint32_t steps, void *stream);
+/* The path tracer's own guide inference (the kernel a guided bounce runs)
+ * on n rays and their closest hits in `scene`, whose guide must be attached:
+ * per query the decoded mixture as a guided vertex stores it, 48 floats:
+ * (log pi_j + log C(kappa_j), kappa_j, mu_j.x, mu_j.y) for lobes j = 0..7,
+ * then mu_j.z, then pi_j (C: the vMF normalisation).  The query of a hit is
+ * the hit position normalised to the scene box, the geometric normal facing
+ * the ray, and the reversed ray direction.  Misses give no output.  For
+ * parity testing; synchronises the stream. */
+wl_status wl_guide_infer_hits(wl_scene scene, const wl_ray *d_rays, const wl_hit *d_hits,
+                              uint32_t n, float *d_decoded, void *stream);
 /* Attaches (mode != 0) or detaches (guide NULL or mode 0) a guide. */
 wl_status wl_scene_set_guide(wl_scene scene, wl_guide guide, uint32_t mode);
 

@@ -10,6 +10,8 @@ one sample.
                         from the bottom row, as the film stores it), centred
                         on the caustic that the glass sphere focuses onto the
                         metal floor
+  caustic_crop128_depth8_fp64
+                        the same window at depth 8
 
 Samples use frame ids 2^30 .. 2^30 + 65535, disjoint from every frame the GPU
 tests and the benchmark render, so the reference is independent of them.
@@ -43,6 +45,9 @@ FIXTURES = {
     "caustic_crop128_fp64": dict(scene="glossy_caustic", width=1024, height=1024,
                                  window=(448, 416, 128, 128), max_depth=16, rr_depth=5,
                                  rr_prob=0.9),
+    "caustic_crop128_depth8_fp64": dict(scene="glossy_caustic", width=1024, height=1024,
+                                        window=(448, 416, 128, 128), max_depth=8, rr_depth=5,
+                                        rr_prob=0.9),
 }
 
 

@@ -184,3 +184,65 @@ def test_training_learns_a_direction_field(gpu_ctx):
     assert loss1 < loss0 - 2.0
     assert np.median(q_true) > 20 * (1 / (4 * np.pi))
     g.close()
+
+
+def test_path_tracer_inference_matches_oracle(gpu_ctx):
+    """The guide inference the path tracer runs at a guided bounce
+    (k_guide_infer_pt: the query built from a ray and its hit, encoding,
+    network, decode across lane groups into the stored per-vertex form)
+    against the oracle's query, forward pass and decode (oracle_guide.c) on
+    camera and bounce rays of the glossy-caustic scene."""
+    import torch
+
+    import wayland_amd as wl
+    from wayland_amd import scenes
+    print("case: k_guide_infer_pt on 3000 rays of the glossy caustic vs oracle", flush=True)
+    sc = scenes.glossy_caustic(64, 64, sphere_subdiv=3, floor_n=20)
+    accel = wl.Accel(gpu_ctx, sc)
+    pt = wl.PathTracer(gpu_ctx, accel, sc)
+    P = random_params(31, out_scale=1.0)
+    g = _guide(gpu_ctx, P)
+    pt.set_guide(g, wl.GUIDE_SAMPLE)
+    rng = np.random.default_rng(5)
+    n = 3000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform([-1.5, 0.05, -1.5], [1.5, 1.0, 1.5], (n, 3))
+    d = rng.normal(size=(n, 3))
+    d[:, 1] = -np.abs(d[:, 1])  # mostly towards the floor and the sphere
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3], rays[:, 7] = 1e-3, 1e30
+    tr = torch.from_numpy(rays).cuda()
+    hits = accel.trace_closest(tr)
+    dec = pt.guide_infer_hits(tr, hits).cpu().numpy()
+    h = wl.hits_to_numpy(hits)
+    tris = sc.triangles().reshape(-1, 3, 3)
+    v = sc.flatten()["vertices"].reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    inv = (1.0 / (hi - lo)).astype(np.float32)
+    ok = h["prim"] != 0xFFFFFFFF
+    assert ok.sum() > n // 2
+    errs = []
+    for i in np.nonzero(ok)[0]:
+        t = tris[h["prim"][i]]
+        ng = np.cross(t[1] - t[0], t[2] - t[0]).astype(np.float64)
+        ng /= np.linalg.norm(ng)
+        if ng @ rays[i, 4:7] > 0:
+            ng = -ng
+        pos = rays[i, 0:3].astype(np.float32) + rays[i, 4:7] * h["t"][i]
+        q = np.concatenate([(pos - lo) * inv, ng, -rays[i, 4:7]]).astype(np.float32)
+        out = ob.guide_forward(P.astype(np.float32).astype(np.float64), q[None], True)[0]
+        pi, kap, mu = ob.guide_decode(out)
+        got = dec[i]
+        gpi, gkap = got[40:48], got[1:32:4]
+        gmu = np.stack([got[2:32:4], got[3:32:4], got[32:40]], 1)
+        errs.append(max(np.abs(gpi - pi).max(), (np.abs(gkap - kap) / (1 + kap)).max(),
+                        np.abs(gmu - mu).max()))
+    errs = np.array(errs)
+    print(f"{len(errs)} hits: max err {errs.max():.3e}, 99.9% {np.quantile(errs, 0.999):.3e}")
+    assert np.quantile(errs, 0.999) < 5e-3
+    assert errs.max() < 5e-2
+    # misses leave their slots untouched (zeros)
+    assert not dec[~ok].any()
+    pt.set_guide(None)
+    g.close()
+    pt.close()

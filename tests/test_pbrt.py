@@ -15,7 +15,7 @@ from wayland_amd import pbrt, scenes
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CBOX = os.path.join(GOLDEN, "cbox.pbrt")
-WATER = "/root/reference/scene/water-caustic/water.pbrt"
+WATER = os.path.join(GOLDEN, "water-caustic", "water.pbrt")
 
 
 def _write(tmp_path, name, text):
@@ -287,7 +287,6 @@ def test_unsupported_input_raises(tmp_path, snippet, exc):
         pbrt.load_pbrt(f)
 
 
-@pytest.mark.skipif(not os.path.exists(WATER), reason="reference scene folder not present")
 def test_water_caustic_scene_loads():
     # scene/water-caustic/water.pbrt: two plymesh water surfaces scaled 1.5x
     # in y, a tiny very bright light; the reference's loader stops at plymesh

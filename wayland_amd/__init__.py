@@ -593,6 +593,18 @@ class PathTracer:
                   nsamples, C.c_void_p(out.data_ptr()), _stream_ptr(film.device))
         return out
 
+    def guide_infer_hits(self, rays, hits):
+        """The path tracer's guide inference on (n, 8) rays and their (n, 4)
+        closest hits: (n, 48) decoded mixtures as a guided vertex stores
+        them (see wl_guide_infer_hits)."""
+        torch = _torch()
+        n = rays.shape[0]
+        out = torch.zeros((n, 48), dtype=torch.float32, device=rays.device)
+        _lib.call("wl_guide_infer_hits", self.h, C.c_void_p(rays.data_ptr()),
+                  C.c_void_p(hits.data_ptr()), n, C.c_void_p(out.data_ptr()),
+                  _stream_ptr(rays.device))
+        return out
+
     def set_guide(self, guide, mode=GUIDE_SAMPLE | GUIDE_TRAIN):
         """Attach a Guide (mode GUIDE_SAMPLE, optionally | GUIDE_TRAIN) or
         detach (guide None).  The guide must outlive the attachment."""

@@ -182,6 +182,7 @@ SIGNATURES = {
                                         C.POINTER(MeshTexture), _U32]),
     "wl_pt_render": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, C.POINTER(PTStats), _VP]),
     "wl_pt_render_noalloc": (C.c_int, [_VP, C.POINTER(PTConfig), _VP, _VP]),
+    "wl_guide_infer_hits": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP]),
     "wl_pt_trace_samples": (C.c_int, [_VP, C.POINTER(PTConfig), C.c_uint32, _VP, _VP,
                                       C.c_uint32, _VP]),
     "wl_pt_render_samples": (C.c_int, [_VP, C.POINTER(PTConfig), _U32, _VP, _VP]),

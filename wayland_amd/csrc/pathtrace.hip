@@ -1170,6 +1170,14 @@ hipError_t pt_arena_copy_nodes(WlBvh8Node *dst, const WlBvh8Node *src, uint32_t 
     return hipGetLastError();
 }
 
+hipError_t pt_guide_infer_queue(const PtParams &P, const PtScene &S, const PtBuffers &B,
+                                uint32_t n, int num_cus, hipStream_t st)
+{
+    const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, (uint32_t)num_cus * 4));
+    hipLaunchKernelGGL(k_guide_infer_pt, dim3(g), dim3(kPtBlock), 0, st, P, S, B, 0);
+    return hipGetLastError();
+}
+
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus, hipStream_t st)
 {
     const uint32_t npaths = P.npix * P.nsamples;

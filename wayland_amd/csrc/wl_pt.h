@@ -186,6 +186,10 @@ hipError_t pt_arena_copy_nodes(WlBvh8Node *dst, const WlBvh8Node *src, uint32_t 
                                uint32_t node0, uint32_t tri0, hipStream_t st);
 // Moves the finished batch's training records with a positive weight into
 // the training buffer.
+// The path tracer's guide inference (k_guide_infer_pt) over queue 0 of B:
+// n = B.counters[0] entries of rays0 / hits, decoded mixtures to B.gout.
+hipError_t pt_guide_infer_queue(const PtParams &P, const PtScene &S, const PtBuffers &B,
+                                uint32_t n, int num_cus, hipStream_t st);
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus,
                             hipStream_t st);
 hipError_t pt_accumulate(const PtParams &P, const PtBuffers &B, float *film,

@@ -775,6 +775,34 @@ wl_status wl_pt_launch_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
     return WL_SUCCESS;
 }
 
+wl_status wl_guide_infer_hits(wl_scene s, const wl_ray *rays, const wl_hit *hits, uint32_t n,
+                              float *decoded, void *stream)
+{
+    if (!s || !rays || !hits || !decoded)
+        return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
+    if (!s->guide || !s->guide_mode)
+        return set_error(WL_ERROR_INVALID_VALUE, "the scene has no guide attached");
+    if (n == 0)
+        return WL_SUCCESS;
+    WL_HIP_CHECK(hipSetDevice(s->ctx->device));
+    hipStream_t hs = (hipStream_t)stream;
+    wl_pt_config c{};
+    c.width = 1;
+    c.height = 1;
+    c.max_depth = 1;
+    c.rr_prob = 1.0f;
+    wl::PtParams P = params_for(s, &c);
+    wl::PtBuffers B{};
+    B.rays0 = const_cast<wl_ray *>(rays);
+    B.hits = const_cast<wl_hit *>(hits);
+    B.counters = s->d_counters; // the kernel reads the queue length from here
+    B.gout = decoded;
+    WL_HIP_CHECK(hipMemcpyAsync(s->d_counters, &n, sizeof n, hipMemcpyHostToDevice, hs));
+    WL_HIP_CHECK(wl::pt_guide_infer_queue(P, scene_view(s), B, n, s->ctx->num_cus, hs));
+    WL_HIP_CHECK(hipStreamSynchronize(hs)); // n is read from host memory
+    return WL_SUCCESS;
+}
+
 wl_status wl_scene_set_guide(wl_scene s, wl_guide g, uint32_t mode)
 {
     if (!s)
