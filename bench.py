@@ -51,7 +51,6 @@ def parse():
     p.add_argument("--guide-spp", type=int, default=64,
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
-    p.add_argument("--guide-ref-spp", type=int, default=4096)
     p.add_argument("--guide-runs", type=int, default=10)
     p.add_argument("--serve-requests", type=int, default=64,
                    help="batched-serving requests (0 = skip)")
@@ -182,23 +181,25 @@ def relmse(img, ref, trim=0.0, mask=None):
 def bench_guide(args, ctx, wl, torch):
     """Learned guiding on the glossy-caustic workload (1024 x 1024, depth 16):
     guided PT (training included in its time) against plain PT given the same
-    wall time, both scored by relMSE against a converged plain-PT reference
-    rendered on the GPU."""
+    wall time, both scored by relMSE against the committed 65,536-spp FP64
+    oracle images: the whole view on a 256 x 256 pixel grid (every 4th pixel;
+    the headline), its floor pixels, and the 128 x 128 caustic window."""
     from wayland_amd import scenes
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fp64_references as fx
     res = args.guide_res
     sc = scenes.glossy_caustic(res, res)
     accel = wl.Accel(ctx, sc)
     pt = wl.PathTracer(ctx, accel, sc)
     depth = 16
-    film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
-    pt.render(wl.pt_config(res, res, spp=1, max_depth=depth), film, stats=False)  # warm
-    film.zero_()
-    t0 = time.perf_counter()
-    ref_spp = args.guide_ref_spp
-    pt.render(wl.pt_config(res, res, spp=ref_spp, max_depth=depth, frame_offset=1 << 24),
-              film, stats=False)
-    ref = pt.resolve(film, ref_spp).cpu().numpy()
-    log(f"guide: reference {ref_spp} spp in {time.perf_counter() - t0:.1f}s")
+    grid = fx.load("caustic_grid256_fp64")
+    crop = fx.load("caustic_crop128_fp64")
+    if grid is None or crop is None or grid[2]["width"] != res:
+        log("guide: FP64 references missing or for another resolution; skipped")
+        return None
+    gx0, gy0, gw, gh = grid[2]["window"]
+    gs = grid[2]["stride"]
+    cx0, cy0, cw, ch = crop[2]["window"]
 
     def progressive(spp, frame0):
         cfg = wl.pt_config(res, res, spp=1, max_depth=depth, max_batch_paths=res * res)
@@ -209,18 +210,19 @@ def bench_guide(args, ctx, wl, torch):
             cfg.frame_offset = frame0 + k
             pt.render(cfg, f, stats=False)
         torch.cuda.synchronize()
-        return pt.resolve(f, spp).cpu().numpy(), time.perf_counter() - t
+        return pt.resolve(f, spp).cpu().numpy().reshape(res, res, 3), time.perf_counter() - t
 
-    # pixels whose camera ray (through the pixel centre) first meets the metal
-    # floor, where the guided vertex is; the glass sphere's pixels carry the
+    # grid pixels whose camera ray (through the pixel centre) first meets the
+    # metal floor, where the guided vertex is; the glass sphere's pixels carry
     # glass-only paths that no guided vertex lies on
     cam = sc.camera
-    c = (np.arange(res, dtype=np.float64) + 0.5) / res * 2 - 1
-    U, V = np.meshgrid(c, c)  # row = y, pixel index = y * res + x
+    xs = (gx0 + gs * np.arange(gw) + 0.5) / res * 2 - 1
+    ys = (gy0 + gs * np.arange(gh) + 0.5) / res * 2 - 1
+    U, V = np.meshgrid(xs, ys)  # row = y
     d = (np.asarray(cam.lookat, np.float64)[None, None] + U[..., None] * np.asarray(cam.right)
          + V[..., None] * np.asarray(cam.up))
     d /= np.linalg.norm(d, axis=-1, keepdims=True)
-    prays = np.zeros((res * res, 8), np.float32)
+    prays = np.zeros((gw * gh, 8), np.float32)
     prays[:, 0:3] = cam.pos
     prays[:, 3] = 1e-3
     prays[:, 4:7] = d.reshape(-1, 3)
@@ -228,7 +230,13 @@ def bench_guide(args, ctx, wl, torch):
     prim = wl.hits_to_numpy(accel.trace_closest(torch.from_numpy(prays).cuda()))["prim"]
     flat = sc.flatten()
     f0, fn = int(flat["mesh_tri_offset"][1]), int(flat["mesh_tri_count"][1])
-    floor = (prim >= f0) & (prim < f0 + fn)
+    floor = ((prim >= f0) & (prim < f0 + fn)).reshape(gh, gw)
+
+    def scores(img):
+        gi = img[gy0::gs, gx0::gs][:gh, :gw]
+        ci = img[cy0:cy0 + ch, cx0:cx0 + cw]
+        return (relmse(gi, grid[0]), relmse(gi, grid[0], mask=floor),
+                relmse(ci, crop[0]))
 
     # allocate the per-path guide buffers outside the timed runs
     warm_guide = wl.Guide(ctx)
@@ -236,21 +244,9 @@ def bench_guide(args, ctx, wl, torch):
     progressive(1, 1 << 28)
     pt.set_guide(None)
     warm_guide.close()
-    # the converged FP64 oracle crop (65,536 spp) at the caustic, when the
-    # view is the fixture's
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    import make_fp64_references as fx
-    crop = fx.load("caustic_crop128_fp64")
-    if crop is not None and crop[2]["width"] != res:
-        crop = None
-
-    def crop_of(img):
-        x0, y0, ww, wh = crop[2]["window"]
-        return img.reshape(res, res, 3)[y0:y0 + wh, x0:x0 + ww]
-
     # independent runs: fresh network each time, training inside the timing
     runs = args.guide_runs
-    g_rel, g_trim, g_floor, g_crop, g_t, steps = [], [], [], [], [], 0
+    g_sc, g_t, steps = [], [], 0
     for r in range(runs):
         guide = wl.Guide(ctx, seed=r + 1)
         pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
@@ -258,11 +254,7 @@ def bench_guide(args, ctx, wl, torch):
         steps = guide.info()["steps"]
         pt.set_guide(None)
         guide.close()
-        g_rel.append(relmse(img_g, ref))
-        g_trim.append(relmse(img_g, ref, 1e-3))
-        g_floor.append(relmse(img_g, ref, mask=floor))
-        if crop is not None:
-            g_crop.append(relmse(crop_of(img_g), crop[0]))
+        g_sc.append(scores(img_g))
         g_t.append(t_g)
     # plain PT per-sample time (after a warm pass), then the same wall time as
     # the guided runs
@@ -270,48 +262,37 @@ def bench_guide(args, ctx, wl, torch):
     _, t_probe = progressive(16, 1 << 26)
     spp_p = max(1, int(round(float(np.mean(g_t)) / (t_probe / 16))))
     for attempt in range(3):
-        p_rel, p_trim, p_floor, p_crop, p_t = [], [], [], [], []
+        p_sc, p_t = [], []
         for r in range(runs):
             img_p, t_p = progressive(spp_p, (1 << 27) + (r << 20))
-            p_rel.append(relmse(img_p, ref))
-            p_trim.append(relmse(img_p, ref, 1e-3))
-            p_floor.append(relmse(img_p, ref, mask=floor))
-            if crop is not None:
-                p_crop.append(relmse(crop_of(img_p), crop[0]))
+            p_sc.append(scores(img_p))
             p_t.append(t_p)
         # equal time within 3 %, else rescale the sample count and rerun
         ratio = float(np.mean(g_t)) / float(np.mean(p_t))
         if abs(ratio - 1.0) <= 0.03:
             break
         spp_p = max(1, int(round(spp_p * ratio)))
+    g_sc, p_sc = np.array(g_sc), np.array(p_sc)
+
+    def side(k):
+        g, p = g_sc[:, k].tolist(), p_sc[:, k].tolist()
+        return {"relmse_guided": g, "relmse_plain": p,
+                "relmse_ratio_equal_time": float(np.mean(g) / np.mean(p)),
+                "relmse_ratio_ci95": bootstrap_ratio_ci(g, p),
+                "relmse_median_ratio": float(np.median(g) / np.median(p))}
+
     out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
            "runs": runs,
            "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)), "adam_steps": steps},
-           "plain": {"spp": spp_p, "seconds": float(np.mean(p_t))}}
-    if crop is not None:
-        x0, y0, ww, wh = crop[2]["window"]
-        out["caustic_crop"] = {
-            "reference": f"FP64 CPU oracle, {crop[2]['spp']} spp, window x {x0}..{x0 + ww - 1}, "
-                         f"y {y0}..{y0 + wh - 1} of the {res}^2 view (the caustic on the floor)",
-            "relmse_guided": g_crop, "relmse_plain": p_crop,
-            "relmse_ratio_equal_time": float(np.mean(g_crop) / np.mean(p_crop)),
-            "relmse_ratio_ci95": bootstrap_ratio_ci(g_crop, p_crop),
-            "relmse_plain_expected": fx.expected_relmse(crop[0], crop[1], spp_p)}
-    out["full_image"] = {
-        "reference": f"plain PT {ref_spp} spp on the GPU (not converged: fireflies of "
-                     "glass-only paths)",
-        "floor_pixels": int(floor.sum()),
-        "relmse_guided": g_rel, "relmse_plain": p_rel,
-        "relmse_trim_0.1pct_guided": g_trim, "relmse_trim_0.1pct_plain": p_trim,
-        "relmse_floor_guided": g_floor, "relmse_floor_plain": p_floor,
-        "relmse_ratio_equal_time": float(np.mean(g_rel) / np.mean(p_rel)),
-        "relmse_ratio_ci95": bootstrap_ratio_ci(g_rel, p_rel),
-        "relmse_trim_ratio_equal_time": float(np.mean(g_trim) / np.mean(p_trim)),
-        "relmse_floor_ratio_equal_time": float(np.mean(g_floor) / np.mean(p_floor))}
-    # headline: the crop against the converged FP64 oracle where present
-    head = out.get("caustic_crop", out["full_image"])
-    out["relmse_ratio_equal_time"] = head["relmse_ratio_equal_time"]
-    out["relmse_ratio_ci95"] = head["relmse_ratio_ci95"]
+           "plain": {"spp": spp_p, "seconds": float(np.mean(p_t))},
+           "reference": f"FP64 CPU oracle, {grid[2]['spp']} spp",
+           "full_view": dict(side(0), pixels=f"{gw}x{gh} grid, every {gs}th pixel of the view"),
+           "floor": dict(side(1), pixels=f"{int(floor.sum())} grid pixels on the metal floor"),
+           "caustic_window": dict(side(2), pixels=f"x {cx0}..{cx0 + cw - 1}, "
+                                                  f"y {cy0}..{cy0 + ch - 1}")}
+    # headline: the whole view against the converged FP64 oracle
+    out["relmse_ratio_equal_time"] = out["full_view"]["relmse_ratio_equal_time"]
+    out["relmse_ratio_ci95"] = out["full_view"]["relmse_ratio_ci95"]
     log(f"guide: {out}")
     return out
 

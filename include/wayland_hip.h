@@ -456,7 +456,8 @@ wl_status wl_server_set_config(wl_server server, const wl_serve_config *cfg);
 wl_status wl_server_render(wl_server server, const wl_render_request *reqs, uint32_t n,
                            wl_serve_stats *stats, double *done_seconds, void *stream);
 /* Times candidate megabatch sizes, shade grids and lane counts on the
- * requests (each cut to max_spp samples, rendered into scratch films), keeps
+ * requests (each cut to max_spp samples, rendered into scratch films), one
+ * parameter at a time (capacity, then shade grid, then lanes), and keeps
  * the fastest. */
 wl_status wl_server_autotune(wl_server server, const wl_render_request *reqs, uint32_t n,
                              uint32_t max_spp, wl_serve_config *chosen);

@@ -215,13 +215,14 @@ void oracle_pt_render_frame_f32(const oracle_pt *pt, const oracle_pt_config *c,
 void oracle_pt_render_f64(const oracle_pt *pt, const oracle_pt_config *c,
                           uint32_t frame0, uint32_t nframes, double *rgb_mean,
                           int nthreads);
-/* The same over the window [x0, x0+ww) x [y0, y0+wh) of the W x H image
- * (same per-pixel seeds as the whole image); rgb_var (optional) gets the
- * per-pixel variance of one sample (unbiased over the nframes samples). */
+/* The same over the ww x wh pixels (x0 + stride i, y0 + stride j) of the
+ * W x H image (stride 1: a window; same per-pixel seeds as the whole
+ * image); rgb_var (optional) gets the per-pixel variance of one sample
+ * (unbiased over the nframes samples). */
 void oracle_pt_render_f64_window(const oracle_pt *pt, const oracle_pt_config *c, uint32_t x0,
-                                 uint32_t y0, uint32_t ww, uint32_t wh, uint32_t frame0,
-                                 uint32_t nframes, double *rgb_mean, double *rgb_var,
-                                 int nthreads);
+                                 uint32_t y0, uint32_t ww, uint32_t wh, uint32_t stride,
+                                 uint32_t frame0, uint32_t nframes, double *rgb_mean,
+                                 double *rgb_var, int nthreads);
 /* Film update of PathTracing.cu:145-158: running mean in a float RGBA buffer
  * with float factors and double temporaries, then ACES -> u8. */
 void oracle_film_update(float *radiance_rgba, uint8_t *color_rgba,

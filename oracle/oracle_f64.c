@@ -31,6 +31,7 @@ typedef struct
     uint32_t frame0, nframes;
     double *out;
     uint32_t x0, y0, ww; /* window origin and width (whole image: 0, 0, W) */
+    uint32_t stride;     /* pixel step inside the window (1: every pixel)  */
     double *var;         /* optional per-pixel variance of one sample      */
 } f64_ctx;
 
@@ -39,7 +40,7 @@ static void f64_task(void *p, uint32_t b, uint32_t e)
     f64_ctx *fc = (f64_ctx *)p;
     for (uint32_t i = b; i < e; ++i)
     {
-        uint32_t x = fc->x0 + i % fc->ww, y = fc->y0 + i / fc->ww;
+        uint32_t x = fc->x0 + fc->stride * (i % fc->ww), y = fc->y0 + fc->stride * (i / fc->ww);
         double acc[3] = { 0, 0, 0 }, acc2[3] = { 0, 0, 0 };
         for (uint32_t f = 0; f < fc->nframes; ++f)
         {
@@ -67,16 +68,16 @@ void oracle_pt_render_f64(const oracle_pt *pt, const oracle_pt_config *c,
                           uint32_t frame0, uint32_t nframes, double *rgb_mean,
                           int nthreads)
 {
-    oracle_pt_render_f64_window(pt, c, 0, 0, c->width, c->height, frame0, nframes, rgb_mean,
-                                NULL, nthreads);
+    oracle_pt_render_f64_window(pt, c, 0, 0, c->width, c->height, 1, frame0, nframes,
+                                rgb_mean, NULL, nthreads);
 }
 
 void oracle_pt_render_f64_window(const oracle_pt *pt, const oracle_pt_config *c, uint32_t x0,
-                                 uint32_t y0, uint32_t ww, uint32_t wh, uint32_t frame0,
-                                 uint32_t nframes, double *rgb_mean, double *rgb_var,
-                                 int nthreads)
+                                 uint32_t y0, uint32_t ww, uint32_t wh, uint32_t stride,
+                                 uint32_t frame0, uint32_t nframes, double *rgb_mean,
+                                 double *rgb_var, int nthreads)
 {
-    f64_ctx fc = { pt, c, frame0, nframes, rgb_mean, x0, y0, ww, rgb_var };
+    f64_ctx fc = { pt, c, frame0, nframes, rgb_mean, x0, y0, ww, stride ? stride : 1, rgb_var };
     if (pt->bvh->ntris == 0)
     {
         memset(rgb_mean, 0, sizeof(double) * 3 * (size_t)ww * wh);

@@ -207,7 +207,8 @@ static int check_pt(void)
         oracle_pt_trace_f32(pt, &c, (k * 5) % W, (k * 11) % H, k + 7, rgb, tr, 2);
     }
     double *mean = malloc(sizeof(double) * 3 * 5 * 4), *var = malloc(sizeof(double) * 3 * 5 * 4);
-    oracle_pt_render_f64_window(pt, &c, W - 5, H - 4, 5, 4, 1000, 6, mean, var, 2);
+    oracle_pt_render_f64_window(pt, &c, W - 5, H - 4, 5, 4, 1, 1000, 6, mean, var, 2);
+    oracle_pt_render_f64_window(pt, &c, 1, 0, 5, 4, 3, 1000, 2, mean, NULL, 2);
     double *full = malloc(sizeof(double) * 3 * W * H);
     oracle_pt_render_f64(pt, &c, 5, 2, full, 4);
     /* film update + ACES on a frame */

@@ -12,6 +12,8 @@ one sample.
                         metal floor
   caustic_crop128_depth8_fp64
                         the same window at depth 8
+  caustic_grid256_fp64  the whole glossy-caustic view at depth 16, sampled
+                        on a 256 x 256 grid: pixels (1 + 4 i, 1 + 4 j)
 
 Samples use frame ids 2^30 .. 2^30 + 65535, disjoint from every frame the GPU
 tests and the benchmark render, so the reference is independent of them.
@@ -48,6 +50,9 @@ FIXTURES = {
     "caustic_crop128_depth8_fp64": dict(scene="glossy_caustic", width=1024, height=1024,
                                         window=(448, 416, 128, 128), max_depth=8, rr_depth=5,
                                         rr_prob=0.9),
+    "caustic_grid256_fp64": dict(scene="glossy_caustic", width=1024, height=1024,
+                                 window=(1, 1, 256, 256), stride=4, max_depth=16, rr_depth=5,
+                                 rr_prob=0.9),
 }
 
 
@@ -83,9 +88,10 @@ def make(name: str, nthreads: int | None = None) -> dict:
     cfg = opt.config(spec["width"], spec["height"], max_depth=spec["max_depth"],
                      rr_depth=spec["rr_depth"], rr_prob=spec["rr_prob"])
     x0, y0, ww, wh = spec["window"]
+    stride = spec.get("stride", 1)
     t = time.time()
     mean, var = opt.render_f64_window(cfg, x0, y0, ww, wh, FRAME0, SPP,
-                                      nthreads or os.cpu_count())
+                                      nthreads or os.cpu_count(), stride=stride)
     secs = time.time() - t
     blob = np.concatenate([mean.astype("<f4").ravel(), var.astype("<f4").ravel()]).tobytes()
     with open(os.path.join(HERE, name + ".f32"), "wb") as f:

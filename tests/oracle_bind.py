@@ -120,7 +120,7 @@ def lib():
                                            C.c_uint32, f64p, C.c_int]
         L.oracle_pt_render_f64_window.argtypes = [
             C.c_void_p, C.POINTER(PTConfig), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
-            C.c_uint32, C.c_uint32, f64p, f64p, C.c_int]
+            C.c_uint32, C.c_uint32, C.c_uint32, f64p, f64p, C.c_int]
         L.oracle_film_update.argtypes = [f32p, u8p, f32p, C.c_uint32, C.c_uint32]
         L.oracle_example_render.argtypes = [C.c_void_p, f32p, f32p, C.POINTER(ExampleConfig),
                                             C.c_uint32, C.c_uint32, f32p, C.c_int]
@@ -413,13 +413,15 @@ class OraclePT:
                                    nthreads or nthreads_default())
         return out
 
-    def render_f64_window(self, cfg, x0, y0, ww, wh, frame0, nframes, nthreads=None):
-        """FP64 mean and per-pixel one-sample variance over a window of the
-        cfg.width x cfg.height image (pixels keep their whole-image seeds)."""
-        assert x0 + ww <= cfg.width and y0 + wh <= cfg.height
+    def render_f64_window(self, cfg, x0, y0, ww, wh, frame0, nframes, nthreads=None,
+                          stride=1):
+        """FP64 mean and per-pixel one-sample variance over the ww x wh
+        pixels (x0 + stride i, y0 + stride j) of the cfg.width x cfg.height
+        image (pixels keep their whole-image seeds; stride 1: a window)."""
+        assert x0 + stride * (ww - 1) < cfg.width and y0 + stride * (wh - 1) < cfg.height
         mean = np.zeros((wh, ww, 3), np.float64)
         var = np.zeros((wh, ww, 3), np.float64)
-        lib().oracle_pt_render_f64_window(self.h, C.byref(cfg), x0, y0, ww, wh, frame0,
+        lib().oracle_pt_render_f64_window(self.h, C.byref(cfg), x0, y0, ww, wh, stride, frame0,
                                           nframes, _p(mean, f64p), _p(var, f64p),
                                           nthreads or nthreads_default())
         return mean, var

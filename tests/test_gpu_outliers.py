@@ -23,45 +23,12 @@ import numpy as np
 import pytest
 
 import oracle_bind as ob
+import pt_replay as rp
 from wayland_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
 W, H, MAXV = 96, 64, 24
-
-
-def _close(a, b, tol=1e-4):
-    return abs(a - b) <= tol * max(abs(a), abs(b), 1e-6)
-
-
-def _classify(gt, ot, g_rgb, o_rgb):
-    """First diverging vertex of two per-vertex records (rows: depth, mesh,
-    pdf, f|cos|/pdf, throughput, -, -, trans) -> (kind, vertex).  With equal
-    records the samples differ after the last recorded vertex: by a whole
-    contribution (a shadow ray or the final extension ray hits on one side
-    and not the other: "visibility"), or by rounding of order 1e-3 or less
-    accumulated along a long path ("rounding")."""
-    n = min(len(gt), len(ot))
-    for k in range(n):
-        g, o = gt[k], ot[k]
-        if int(g[0]) != int(o[0]) or int(g[1]) != int(o[1]):
-            return "hit", k          # a different surface (or depth) at this vertex
-        if not _close(g[4], o[4]):
-            return "drift", k        # throughput apart with the same branches so far
-        if not (_close(g[2], o[2]) and _close(g[3], o[3])):
-            return "lobe", k         # same vertex, the sampled direction differs
-    if len(gt) != len(ot):
-        return "end", n              # one path ends earlier (RR, pdf, light, cap)
-    rel = np.abs(g_rgb - o_rgb) / np.maximum(np.abs(o_rgb), 1e-6)
-    return ("visibility" if rel.max() > 1e-3 else "rounding"), n
-
-
-def _records(tr):
-    """Rows of a per-vertex record up to its first empty slot."""
-    n = 0
-    while n < tr.shape[0] and tr[n, 0] != 0:
-        n += 1
-    return tr[:n]
 
 
 def test_disney_outliers_are_branch_flips(gpu_ctx):
@@ -73,39 +40,24 @@ def test_disney_outliers_are_branch_flips(gpu_ctx):
     opt = ob.OraclePT(sc)
     cfg = wl.pt_config(W, H, max_depth=16, rr_depth=4)
     ocfg = opt.config(W, H, max_depth=16, rr_depth=4)
-    kinds = {}
-    diffs = []
-    n_out = 0
-    for frame in range(8):
-        g, tr = pt.trace_samples(cfg, frame, MAXV)
-        g = g.cpu().numpy()
-        tr = tr.cpu().numpy()
-        # the traced render is the render: same samples as wl_pt_render_samples
-        g2 = pt.render_samples(cfg, frame).cpu().numpy()
-        assert np.array_equal(g.view(np.uint32), g2.view(np.uint32))
-        o, _ = opt.render_frame(ocfg, frame)
-        diffs.append((g - o).reshape(-1, 3))
-        rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-6)
-        ys, xs = np.nonzero(rel.max(-1) > 1e-5)
-        for y, x in zip(ys, xs):
-            _, otr = opt.trace(ocfg, int(x), int(y), frame, MAXV)
-            gtr = _records(tr[y, x])
-            # the oracle records at most MAXV vertices too
-            kind, k = _classify(gtr, otr, g[y, x], o[y, x])
-            kinds.setdefault(kind, []).append((frame, int(x), int(y), k))
-            n_out += 1
-    total = 8 * W * H
+    # the traced render is the render: same samples as wl_pt_render_samples
+    for frame in (0, 5):
+        g, _ = pt.trace_samples(cfg, frame, MAXV)
+        g2 = pt.render_samples(cfg, frame)
+        assert np.array_equal(g.cpu().numpy().view(np.uint32),
+                              g2.cpu().numpy().view(np.uint32))
+    kinds, total, d = rp.replay_frames(pt, opt, cfg, ocfg, range(8), MAXV)
+    n_out = sum(len(v) for v in kinds.values())
     print(f"outliers {n_out} of {total} samples ({n_out / total:.5f}): " +
           ", ".join(f"{k} {len(v)}" for k, v in sorted(kinds.items())))
     for k in ("drift", "rounding"):
-        for f, x, y, v in kinds.get(k, [])[:5]:
+        for f, x, y, v, _ in kinds.get(k, [])[:5]:
             print(f"  {k}: frame {f} pixel ({x}, {y}) vertex {v}")
     # every outlier is a discrete flip after an agreeing prefix, or rounding
     # below 1e-3 along an otherwise identical path
     assert not kinds.get("drift"), kinds.get("drift")[:10]
     assert n_out / total < 0.005
-    d = np.concatenate(diffs).astype(np.float64)
-    z = d.mean(0) / np.maximum(d.std(0) / np.sqrt(len(d)), 1e-30)
+    z = rp.paired_z(d)
     print(f"paired z of GPU - oracle over 8 frames: {np.round(z, 2)}")
     assert np.abs(z).max() < 4.0
     pt.close()

@@ -152,49 +152,46 @@ def test_water_caustic_file_per_sample_parity(gpu_ctx):
     """The reference's scene file scene/water-caustic/water.pbrt (CC0, a
     fixture under tests/golden/water-caustic/, with its two binary PLY water
     surfaces: 10 meshes, 103,487 triangles) rendered per sample against the
-    oracle at 64 x 64, with the bars of the look-alike above: per sample at
-    depth 3 and on the depth-16 samples whose oracle path never meets the
-    water; zero-mean GPU-minus-oracle differences over 16 frames for the
-    rest (the eta 1.8, roughness 0.001 surface makes long paths
-    ill-conditioned)."""
+    oracle at 64 x 64.  At depth 3 the bar is per sample.  At depth 16 the
+    eta 1.8, roughness 0.001 water (which covers most of the view) makes
+    long paths ill-conditioned, so every sample outside 1e-5 is replayed
+    vertex by vertex against the oracle's trace (tests/pt_replay.py): each
+    must be a branch flip after an agreeing prefix (a different surface,
+    lobe, path end or shadow-ray outcome) or rounding below 1e-3.  A drift
+    of the throughput along the same branches is allowed only after the
+    path has met the water, whose near-specular refraction amplifies
+    ulp-level direction differences (a slightly different point of the same
+    wavy mesh); on every other path it fails.  The GPU-minus-oracle
+    differences must have zero mean (paired z)."""
     import wayland_amd as wl
     from wayland_amd import pbrt
+    import pt_replay as rp
     print("case: water.pbrt (103,487 tris, plymesh water) 64x64", flush=True)
     s = pbrt.load_pbrt(os.path.join(GOLDEN, "water-caustic", "water.pbrt"), 64, 64)
     sc = s.scene
     assert len(sc.meshes) == 10 and sc.num_triangles == 103487
-    water = {len(sc.meshes) - 2, len(sc.meshes) - 1}
     accel = wl.Accel(gpu_ctx, sc)
     pt = wl.PathTracer(gpu_ctx, accel, sc)
     opt = ob.OraclePT(sc)
-
-    def both(depth, frame):
-        g = pt.render_samples(wl.pt_config(64, 64, max_depth=depth, rr_depth=4), frame)
-        o, _ = opt.render_frame(opt.config(64, 64, max_depth=depth, rr_depth=4), frame)
-        return g.cpu().numpy(), o
-
     for frame in range(3):
-        frac = _frac_within(*both(3, frame))
+        g = pt.render_samples(wl.pt_config(64, 64, max_depth=3, rr_depth=4), frame)
+        o, _ = opt.render_frame(opt.config(64, 64, max_depth=3, rr_depth=4), frame)
+        frac = _frac_within(g.cpu().numpy(), o)
         print(f"depth 3 frame {frame}: within 1e-5 {frac:.5f}", flush=True)
         assert frac >= 0.999
-    cfg = opt.config(64, 64, max_depth=16, rr_depth=4)
-    diffs = []
-    for frame in range(16):
-        g, o = both(16, frame)
-        assert np.isfinite(g).all()
-        err = (np.abs(g - o) / np.maximum(np.abs(o), 1e-6)).reshape(-1, 3).max(1)
-        frac = float((err <= REL).mean())
-        diffs.append((g.astype(np.float64) - o).reshape(-1, 3))
-        if frame < 3:
-            dry = np.array([not np.isin(opt.trace(cfg, i % 64, i // 64, frame)[1][:, 1],
-                                        list(water)).any() for i in range(64 * 64)])
-            dry_frac = float((err[dry] <= REL).mean())
-            print(f"depth 16 frame {frame}: within 1e-5 {frac:.5f}, on the {dry.sum()} "
-                  f"samples that never meet the water {dry_frac:.5f}", flush=True)
-            assert dry_frac >= 0.999
-        assert frac >= 0.95
-    d = np.concatenate(diffs)
-    z = d.mean(0) / np.maximum(d.std(0) / np.sqrt(len(d)), 1e-30)
+    cfg = wl.pt_config(64, 64, max_depth=16, rr_depth=4)
+    ocfg = opt.config(64, 64, max_depth=16, rr_depth=4)
+    kinds, total, d = rp.replay_frames(pt, opt, cfg, ocfg, range(16), 24)
+    n_out = sum(len(v) for v in kinds.values())
+    print(f"depth 16: outliers {n_out} of {total} samples ({n_out / total:.4f}): " +
+          ", ".join(f"{k} {len(v)}" for k, v in sorted(kinds.items())))
+    water = {len(sc.meshes) - 2, len(sc.meshes) - 1}
+    dry_drift = [e for e in kinds.get("drift", []) if not water & set(e[4])]
+    print(f"drifts after the water {len(kinds.get('drift', [])) - len(dry_drift)}, "
+          f"elsewhere {len(dry_drift)}")
+    assert not dry_drift, dry_drift[:10]
+    assert n_out / total < 0.08
+    z = rp.paired_z(d)
     print(f"paired z of GPU - oracle over 16 frames: {np.round(z, 2)}")
     assert np.abs(z).max() < 4.0
     pt.close()

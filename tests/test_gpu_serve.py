@@ -121,3 +121,30 @@ def test_serve_rejects_oversized_request(gpu_ctx):
     with pytest.raises(wl.WlError):
         server.render([{"pt": tracers[0][0], "width": 64, "height": 64, "spp": 1}])
     server.close()
+
+
+def test_serve_rejects_overlapping_films(gpu_ctx):
+    """Two requests whose films share pixels would race (their megabatches
+    update both films concurrently): rejected before any work is enqueued,
+    with every film left untouched."""
+    import torch
+
+    import wayland_amd as wl
+    tracers = _tracers(gpu_ctx, 32)
+    server = wl.Server(gpu_ctx, batch_paths=1 << 14)
+    base = torch.zeros((2 * 32 * 32, 6), dtype=torch.float32, device="cuda")
+    same = [{"pt": tracers[0][0], "width": 32, "height": 32, "spp": 1, "film": base[:1024]},
+            {"pt": tracers[1][0], "width": 32, "height": 32, "spp": 1, "film": base[:1024]}]
+    part = [{"pt": tracers[0][0], "width": 32, "height": 32, "spp": 1, "film": base[:1024]},
+            {"pt": tracers[2][0], "width": 32, "height": 32, "spp": 1, "film": base[512:1536]}]
+    for reqs in (same, part):
+        with pytest.raises(wl.WlError) as e:
+            server.render(reqs)
+        assert e.value.status == 1  # WL_ERROR_INVALID_VALUE
+    assert not bool(base.any())
+    # adjacent, disjoint films are fine
+    ok = [{"pt": tracers[0][0], "width": 32, "height": 32, "spp": 1, "film": base[:1024]},
+          {"pt": tracers[1][0], "width": 32, "height": 32, "spp": 1, "film": base[1024:]}]
+    server.render(ok)
+    assert bool(base[:1024, :3].any()) and bool(base[1024:, :3].any())
+    server.close()

@@ -15,38 +15,68 @@ namespace wl
 {
 
 constexpr int kTraceBlock = 256;
+// The first nodes of the BVH (laid out breadth first: the root, its group of
+// 8 and their groups, levels 0-2 of a full tree) are copied to LDS per
+// workgroup; every ray starts there, so about half of a ray's node visits
+// read LDS instead of the L1/L2 path.  9,344 bytes beside the 16-entry stack
+// keep 6 workgroups (24 waves) per CU; the 32-entry-stack kernels for deep
+// BVHs have no room for it.
+constexpr uint32_t kTopNodes = 73;
+
+template <int STACK>
+constexpr uint32_t top_nodes() { return STACK == 16 ? kTopNodes : 0u; }
+
+// Copies nodes [0, ntop) to LDS (all threads of the workgroup, then a barrier).
+__device__ __forceinline__ void load_top(uint4 *top, const WlBvh8Node *__restrict__ nodes,
+                                         uint32_t ntop)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(nodes);
+    for (uint32_t i = threadIdx.x; i < 8u * ntop; i += kTraceBlock)
+        top[i] = src[i];
+    __syncthreads();
+}
 
 __device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32_t i, TravRay &r)
 {
     load_ray_nt(rays, i, r);
 }
 
+// ntop: nodes copied to LDS, min(node count, top_nodes<STACK>()).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
 void k_trace_closest_p(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
-    const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits)
+    const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits, uint32_t ntop)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
+    __shared__ uint4 top[8 * (top_nodes<STACK>() ? top_nodes<STACK>() : 1)];
+    if (top_nodes<STACK>())
+        load_top(top, nodes, ntop);
     trav_persistent<false, kTraceBlock, false, false, true, 4, 24>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
-        [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); });
+        [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); },
+        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u);
 }
 
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 void k_trace_any_p(const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
-                   const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits)
+                   const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits,
+                   uint32_t ntop)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
+    __shared__ uint4 top[8 * (top_nodes<STACK>() ? top_nodes<STACK>() : 1)];
+    if (top_nodes<STACK>())
+        load_top(top, nodes, ntop);
     trav_persistent<true, kTraceBlock, false, false, true>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {
             if (h.prim != WL_INVALID)
                 atomicOr(&bits[i >> 5], 1u << (i & 31u));
-        });
+        },
+        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u);
 }
 
 // Instrumented closest hit: one ray per lane, per-ray node/triangle counts.
@@ -113,7 +143,8 @@ static hipError_t closest_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t
     const uint32_t grid = (uint32_t)std::min<uint64_t>(
         grid_cache, ((uint64_t)n + kTraceBlock - 1) / kTraceBlock);
     hipLaunchKernelGGL(k_trace_closest_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s,
-                       a->d_nodes, a->d_tris, rays, n, hits);
+                       a->d_nodes, a->d_tris, rays, n, hits,
+                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()));
     return hipGetLastError();
 }
 
@@ -134,7 +165,8 @@ static hipError_t any_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t n,
     hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>((words + 255) / 256, 4096)),
                        dim3(256), 0, s, bits, words);
     hipLaunchKernelGGL(k_trace_any_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s, a->d_nodes,
-                       a->d_tris, rays, n, bits);
+                       a->d_tris, rays, n, bits,
+                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()));
     return hipGetLastError();
 }
 

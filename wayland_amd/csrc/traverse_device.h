@@ -214,12 +214,38 @@ struct NodeHits
     uint32_t child_base, tri_base, inner, trimask;
 };
 
+// top (optional): a copy of nodes [0, ntop) in LDS, read instead of memory
+// for those nodes (the builder lays the top levels out first).
 WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
-                          const TravRay &r, float t_best)
+                          const TravRay &r, float t_best, const WlBvh8Node *top = nullptr,
+                          uint32_t ntop = 0)
 {
-    const uint4 *np = reinterpret_cast<const uint4 *>(nodes + node);
-    const uint4 h0 = np[0];
-    const uint4 h1 = np[1];
+    // near/far quantised planes chosen by the ray's direction signs
+    // (lo planes at words 2-4, hi planes at 5-7: a per-lane offset picks them)
+    const uint32_t nx = r.code & 1u, ny = (r.code >> 1) & 1u, nz = (r.code >> 2) & 1u;
+    uint4 h0, h1, nqx, fqx, nqy, fqy, nqz, fqz;
+    auto fetch = [&](auto np) {
+        h0 = np[0];
+        h1 = np[1];
+        nqx = np[2 + 3 * nx];
+        fqx = np[5 - 3 * nx];
+        nqy = np[3 + 3 * ny];
+        fqy = np[6 - 3 * ny];
+        nqz = np[4 + 3 * nz];
+        fqz = np[7 - 3 * nz];
+    };
+#if defined(__HIP_DEVICE_COMPILE__)
+    // distinct address spaces keep the two paths apart: LDS reads for the
+    // cached nodes, global loads for the rest (not one generic-pointer load)
+    typedef __attribute__((address_space(3))) const uint4 lds_u4;
+    typedef __attribute__((address_space(1))) const uint4 glb_u4;
+    if (node < ntop)
+        fetch((lds_u4 *)(reinterpret_cast<const uint4 *>(top + node)));
+    else
+        fetch((glb_u4 *)(reinterpret_cast<const uint4 *>(nodes + node)));
+#else
+    fetch(reinterpret_cast<const uint4 *>(node < ntop ? top + node : nodes + node));
+#endif
     const float px = wl_u2f(h0.x), py = wl_u2f(h0.y), pz = wl_u2f(h0.z);
     const uint32_t ew = h0.w;
     const float sx = wl_u2f((ew & 0xFFu) << 23);
@@ -232,12 +258,6 @@ WL_HD NodeHits node_visit(const WlBvh8Node *__restrict__ nodes, uint32_t node,
     const float ay = (py - r.o[1]) * r.inv[1];
     const float az = (pz - r.o[2]) * r.inv[2];
     const float six = sx * r.inv[0], siy = sy * r.inv[1], siz = sz * r.inv[2];
-    // near/far quantised planes chosen by the ray's direction signs
-    // (lo planes at words 2-4, hi planes at 5-7: a per-lane offset picks them)
-    const uint32_t nx = r.code & 1u, ny = (r.code >> 1) & 1u, nz = (r.code >> 2) & 1u;
-    const uint4 nqx = np[2 + 3 * nx], fqx = np[5 - 3 * nx];
-    const uint4 nqy = np[3 + 3 * ny], fqy = np[6 - 3 * ny];
-    const uint4 nqz = np[4 + 3 * nz], fqz = np[7 - 3 * nz];
     const float cull = t_best * 1.00001f;
     const float tmin = r.tmin;
     const uint32_t nxw[4] = { nqx.x, nqx.y, nqx.z, nqx.w };
@@ -369,13 +389,13 @@ template <bool ANY, bool COUNT, int BLOCK, bool AUX = false>
 WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
                      const WlTriRecord *__restrict__ tris, const TravRay &r,
                      TravState &st, uint32_t *lds_stack_lane, uint32_t &n_nodes,
-                     uint32_t &n_tris)
+                     uint32_t &n_tris, const WlBvh8Node *top = nullptr, uint32_t ntop = 0)
 {
     uint32_t trimask = 0, tri_base = 0;
     if (st.visit)
     {
         WL_TRAV_CHECK(st.node < wl_trav_nodes_bound, "node index");
-        const NodeHits h = node_visit(nodes, st.node, r, st.res.t);
+        const NodeHits h = node_visit(nodes, st.node, r, st.res.t, top, ntop);
         if (COUNT)
             ++n_nodes;
         trimask = h.trimask;
@@ -410,7 +430,8 @@ WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
 template <int BLOCK, bool AUX = false>
 WL_HD bool trav_step_one(const WlBvh8Node *__restrict__ nodes,
                          const WlTriRecord *__restrict__ tris, const TravRay &r,
-                         TravState &st, uint32_t *lds_stack_lane)
+                         TravState &st, uint32_t *lds_stack_lane,
+                         const WlBvh8Node *top = nullptr, uint32_t ntop = 0)
 {
     if (st.trimask)
     {
@@ -428,7 +449,7 @@ WL_HD bool trav_step_one(const WlBvh8Node *__restrict__ nodes,
             return true;
     }
     WL_TRAV_CHECK(st.node < wl_trav_nodes_bound, "node index");
-    const NodeHits h = node_visit(nodes, st.node, r, st.res.t);
+    const NodeHits h = node_visit(nodes, st.node, r, st.res.t, top, ntop);
     st.trimask = h.trimask;
     st.tri_base = h.tri_base;
     st.grp_base = h.child_base;

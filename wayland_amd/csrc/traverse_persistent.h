@@ -72,12 +72,16 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // rays, 8 for multi-request queues, 2 for the path tracer's shadow rays); the
 // path tracer's extension rays keep contiguous ranges, which keep its
 // coherent queues together.
+// top / ntop (optional): nodes [0, ntop) copied to LDS by the caller (see
+// node_visit).
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
           uint32_t CHUNKS = 0, int REFILL = kRefillLanes, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
                                                 const WlTriRecord *__restrict__ tris,
                                                 uint32_t n, uint32_t *stack_lane,
-                                                Fetch fetch, Emit emit)
+                                                Fetch fetch, Emit emit,
+                                                const WlBvh8Node *top = nullptr,
+                                                uint32_t ntop = 0)
 {
     static_assert(BLOCK % 64 == 0, "whole waves only");
     const uint32_t waves = gridDim.x * (BLOCK / 64);
@@ -145,9 +149,10 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
         if (alive)
         {
             if constexpr (ANY)
-                fin = trav_step<ANY, false, BLOCK, AUX>(nodes, tris, r, st, stack_lane, nn, nt);
+                fin = trav_step<ANY, false, BLOCK, AUX>(nodes, tris, r, st, stack_lane, nn, nt,
+                                                        top, ntop);
             else
-                fin = trav_step_one<BLOCK, AUX>(nodes, tris, r, st, stack_lane);
+                fin = trav_step_one<BLOCK, AUX>(nodes, tris, r, st, stack_lane, top, ntop);
         }
         if (fin)
         {

@@ -695,35 +695,63 @@ wl_status wl_server_autotune(wl_server s, const wl_render_request *reqs, uint32_
     double best_rate = -1.0;
     const uint32_t batches[] = { 1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25,
                                  1u << 26 };
-    const uint32_t shades[] = { 4, 8, 16 };
     wl_status st = WL_SUCCESS;
-    for (uint32_t bp : batches)
-    {
-        if (bp < max_npix)
-            continue;
-        for (uint32_t cand = 0; cand < 6; ++cand)
-        {
-            wl_serve_config c = saved;
-            c.batch_paths = bp;
-            c.shade_blocks_per_cu = shades[cand % 3];
-            c.lanes = 1 + cand / 3;
-            c.use_graph = 1;
-            s->cfg = c;
-            wl_serve_stats a{}, b{};
-            st = run(s, probe.data(), n, &a, nullptr, nullptr); // warm (capture)
-            if (st == WL_SUCCESS)
-                st = run(s, probe.data(), n, &b, nullptr, nullptr);
-            if (st != WL_SUCCESS)
-                break;
-            const double rate = (double)b.paths / std::max(1e-9, b.seconds);
-            if (rate > best_rate)
-            {
-                best_rate = rate;
-                best = c;
-            }
-        }
+    // times one candidate (a warm run that captures its graph, then a timed
+    // one); returns its rate in paths per second (0 on failure)
+    auto trial = [&](const wl_serve_config &c) {
+        s->cfg = c;
+        wl_serve_stats a{}, b{};
+        st = run(s, probe.data(), n, &a, nullptr, nullptr);
+        if (st == WL_SUCCESS)
+            st = run(s, probe.data(), n, &b, nullptr, nullptr);
         if (st != WL_SUCCESS)
+            return 0.0;
+        const double rate = (double)b.paths / std::max(1e-9, b.seconds);
+        if (rate > best_rate)
+        {
+            best_rate = rate;
+            best = c;
+        }
+        return rate;
+    };
+    // coordinate search: the megabatch capacity first (8 shade workgroups per
+    // CU, two lanes when there are two requests), then the shade grid at the
+    // best capacity, then the other lane count -- at most 7 + 2 + 1
+    // candidates instead of the 42 of the full grid
+    wl_serve_config c = saved;
+    c.use_graph = 1;
+    c.shade_blocks_per_cu = 8;
+    c.lanes = n >= 2 ? 2u : 1u;
+    // the probe's spp is capped, so capacities beyond its size time alike; the
+    // full workload has more samples and fills larger megabatches: take the
+    // largest capacity within 2 % of the fastest
+    double rate_of[7] = {};
+    for (int k = 0; k < 7; ++k)
+    {
+        if (batches[k] < max_npix || st != WL_SUCCESS)
+            continue;
+        c.batch_paths = batches[k];
+        rate_of[k] = trial(c);
+    }
+    for (int k = 6; k >= 0 && st == WL_SUCCESS && best_rate > 0; --k)
+        if (rate_of[k] >= 0.98 * best_rate)
+        {
+            best.batch_paths = batches[k];
             break;
+        }
+    for (uint32_t sh : { 4u, 16u })
+    {
+        if (st != WL_SUCCESS || best_rate <= 0)
+            break;
+        c = best;
+        c.shade_blocks_per_cu = sh;
+        trial(c);
+    }
+    if (st == WL_SUCCESS && best_rate > 0 && n >= 2)
+    {
+        c = best;
+        c.lanes = best.lanes == 2 ? 1u : 2u;
+        trial(c);
     }
     hipFree(scratch);
     s->cfg = st == WL_SUCCESS && best_rate > 0 ? best : saved;
