@@ -147,12 +147,38 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
             B.sort_bins[b] = 0;
 }
 
+// ---- top BVH levels in LDS ---------------------------------------------------
+// The single-scene traversal kernels with the 16-entry stack copy the first
+// 73 nodes (levels 0-2 of the breadth-first node layout: root, its group and
+// their groups) to LDS; every ray starts there.  9,344 bytes beside the
+// stack keep 6 workgroups per CU.  Multi-request launches start rays at other
+// roots of the node arena and keep the plain loads.
+constexpr uint32_t kPtTopNodes = 73;
+
+template <int STACK, bool MULTI>
+constexpr uint32_t pt_top_nodes() { return (STACK == 16 && !MULTI) ? kPtTopNodes : 0u; }
+
+template <int STACK, bool MULTI>
+__device__ __forceinline__ uint32_t pt_load_top(uint4 *top, const PtScene &S)
+{
+    if (!pt_top_nodes<STACK, MULTI>())
+        return 0u;
+    const uint32_t ntop = min(S.num_nodes, pt_top_nodes<STACK, MULTI>());
+    const uint4 *src = reinterpret_cast<const uint4 *>(S.nodes);
+    for (uint32_t i = threadIdx.x; i < 8u * ntop; i += kPtBlock)
+        top[i] = src[i];
+    __syncthreads();
+    return ntop;
+}
+
 // ---- extend: closest hit for the active queue -----------------------------
 template <int STACK, bool MULTI>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(STACK == 16 ? 6 : 5)))
 void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
+    __shared__ uint4 top[8 * (pt_top_nodes<STACK, MULTI>() ? pt_top_nodes<STACK, MULTI>() : 1)];
+    const uint32_t ntop = pt_load_top<STACK, MULTI>(top, S);
     const uint32_t count = B.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
@@ -174,7 +200,8 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
             store);
     else
         trav_persistent<false, kPtBlock, false, false, false, 0, 20>(
-            S.nodes, S.tris, count, stack + threadIdx.x, load, store);
+            S.nodes, S.tris, count, stack + threadIdx.x, load, store,
+            reinterpret_cast<const WlBvh8Node *>(top), ntop);
 }
 
 // ---- material sort of the shade queue ------------------------------------------
@@ -752,6 +779,8 @@ __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(STACK 
 void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
+    __shared__ uint4 top[8 * (pt_top_nodes<STACK, MULTI>() ? pt_top_nodes<STACK, MULTI>() : 1)];
+    const uint32_t ntop = pt_load_top<STACK, MULTI>(top, S);
     const uint32_t count = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0)
         B.stats[1] += count;
@@ -786,7 +815,8 @@ void k_shadow(PtScene S, PtBuffers B, const PtReq *__restrict__ reqs)
             add);
     else
         trav_persistent<true, kPtBlock, false, false, false, 2, 20>(
-            S.nodes, S.tris, count, stack + threadIdx.x, load, add);
+            S.nodes, S.tris, count, stack + threadIdx.x, load, add,
+            reinterpret_cast<const WlBvh8Node *>(top), ntop);
 }
 
 // ---- batched serving: raygen and film update of a multi-request megabatch ----

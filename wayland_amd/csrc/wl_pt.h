@@ -55,6 +55,7 @@ struct PtScene
     const PtTexture *textures;
     const uint32_t *texels;
     uint32_t bvh_depth;
+    uint32_t num_nodes; // BVH8 nodes (the top ones are copied to LDS by traversal)
 };
 
 // One batch = pixels [pix0, pix0 + npix) x samples [frame0, frame0 + nsamples)

@@ -73,6 +73,7 @@ PtScene pt_scene_view(wl_scene s)
     S.textures = (const PtTexture *)s->d_textures;
     S.texels = s->d_texels;
     S.bvh_depth = s->accel->info.depth;
+    S.num_nodes = s->accel->num_nodes;
     return S;
 }
 
