@@ -183,7 +183,8 @@ def bench_guide(args, ctx, wl, torch):
     guided PT (training included in its time) against plain PT given the same
     wall time, both scored by relMSE against the committed 65,536-spp FP64
     oracle images: the whole view on a 256 x 256 pixel grid (every 4th pixel;
-    the headline), its floor pixels, and the 128 x 128 caustic window."""
+    the headline), its floor pixels, its pixels off and on the glass sphere,
+    and the 128 x 128 caustic window."""
     from wayland_amd import scenes
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import make_fp64_references as fx
@@ -231,12 +232,15 @@ def bench_guide(args, ctx, wl, torch):
     flat = sc.flatten()
     f0, fn = int(flat["mesh_tri_offset"][1]), int(flat["mesh_tri_count"][1])
     floor = ((prim >= f0) & (prim < f0 + fn)).reshape(gh, gw)
+    s0, sn = int(flat["mesh_tri_offset"][0]), int(flat["mesh_tri_count"][0])
+    sphere = ((prim >= s0) & (prim < s0 + sn)).reshape(gh, gw)
 
     def scores(img):
         gi = img[gy0::gs, gx0::gs][:gh, :gw]
         ci = img[cy0:cy0 + ch, cx0:cx0 + cw]
         return (relmse(gi, grid[0]), relmse(gi, grid[0], mask=floor),
-                relmse(ci, crop[0]))
+                relmse(ci, crop[0]), relmse(gi, grid[0], mask=~sphere),
+                relmse(gi, grid[0], mask=sphere))
 
     # allocate the per-path guide buffers outside the timed runs
     warm_guide = wl.Guide(ctx)
@@ -289,7 +293,17 @@ def bench_guide(args, ctx, wl, torch):
            "full_view": dict(side(0), pixels=f"{gw}x{gh} grid, every {gs}th pixel of the view"),
            "floor": dict(side(1), pixels=f"{int(floor.sum())} grid pixels on the metal floor"),
            "caustic_window": dict(side(2), pixels=f"x {cx0}..{cx0 + cw - 1}, "
-                                                  f"y {cy0}..{cy0 + ch - 1}")}
+                                                  f"y {cy0}..{cy0 + ch - 1}"),
+           "off_sphere": dict(side(3), pixels=f"{int((~sphere).sum())} grid pixels whose "
+                                              "camera ray misses the glass sphere"),
+           "sphere": dict(side(4), pixels=f"{int(sphere.sum())} grid pixels whose camera "
+                                          "ray meets the glass sphere first")}
+    # share of the whole-view error that the sphere's pixels carry (mean over
+    # runs of sum over sphere pixels / sum over all pixels)
+    frac = float(sphere.mean())
+    out["sphere_share_of_error"] = {
+        "guided": float(np.mean(g_sc[:, 4] * frac / g_sc[:, 0])),
+        "plain": float(np.mean(p_sc[:, 4] * frac / p_sc[:, 0]))}
     # headline: the whole view against the converged FP64 oracle
     out["relmse_ratio_equal_time"] = out["full_view"]["relmse_ratio_equal_time"]
     out["relmse_ratio_ci95"] = out["full_view"]["relmse_ratio_ci95"]

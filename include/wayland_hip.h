@@ -317,7 +317,9 @@ wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
  * mixture over directions.  With a guide attached to a scene the tracer picks
  * the guide with probability mix_alpha at vertices whose material has
  * trans = 0 and clearcoat = 0, else the reference BSDF sampler, and weights
- * by the mixed density (one-sample MIS), so the estimate stays unbiased.  In
+ * by the mixed density (one-sample MIS), so the estimate stays unbiased.
+ * Only vertices on bounces below guided_bounces (0: every bounce) that are
+ * among the path's first guided_vertices such vertices (0: all) are guided.  In
  * training mode every path records its first guided vertex, and after each
  * batch the recorded samples train the network for steps_per_batch Adam
  * steps on the KL objective -F/p log q (Mueller et al., TOG 2019).
@@ -345,7 +347,8 @@ typedef struct wl_guide_config
     float mix_alpha;          /* probability of sampling the guide            */
     float max_weight;         /* training weights are clamped to this         */
     uint32_t guided_bounces;  /* guide the first bounces only (0 = every one) */
-    uint32_t pad_;
+    uint32_t guided_vertices; /* guide a path's first guidable vertices only
+                                 (0 = every one)                              */
 } wl_guide_config;
 
 typedef struct wl_guide_info

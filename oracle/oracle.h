@@ -256,9 +256,10 @@ void oracle_example_render(const oracle_pt *pt, const float *mesh_color,
  * 32 for l = 4; in = 64.
  *
  * Query (9 floats): position normalised to [0,1]^3, normal, outgoing
- * direction.  Input encoding x[64]: for e < 36, frequency f = e / 6, axis
- * d = (e % 6) / 2: sin (e even) or cos (e odd) of 2^f * pi * p[d];
- * x[36..38] = normal, x[39..41] = outgoing direction, x[42..63] = 0.
+ * direction.  Input encoding x[64], with g = (e % 16) / 4 and pair
+ * p = 2 (e / 16) + (e % 4) / 2: for g < 3 and p < 6, sin (e even) or cos
+ * (e odd) of 2^p * pi * position[g]; for g < 3 and p = 6, normal[g] (e even)
+ * or outgoing direction[g] (e odd); every other x[e] = 0.
  *
  * Output row r belongs to lobe 2 * ((r % 16) / 4) + r / 16, parameter r % 4:
  * 0 = mixture logit a, 1 = raw concentration k (kappa = softplus(k)),

@@ -58,17 +58,18 @@ static double f32(double x, int emulate) { return emulate ? (double)(float)x : x
 void oracle_guide_encode(const float q[9], double x[64])
 {
     for (int e = 0; e < 64; ++e)
+    {
+        const int g = (e % 16) / 4, p = 2 * (e / 16) + (e % 4) / 2, odd = e % 2;
         x[e] = 0.0;
-    for (int e = 0; e < 36; ++e)
-    {
-        const int f = e / 6, d = (e % 6) / 2;
-        const double a = ldexp(PI_D, f) * (double)q[d];
-        x[e] = (e % 2 == 0) ? sin(a) : cos(a);
-    }
-    for (int k = 0; k < 3; ++k)
-    {
-        x[36 + k] = q[3 + k];
-        x[39 + k] = q[6 + k];
+        if (g == 3)
+            continue;
+        if (p < 6)
+        {
+            const double a = ldexp(PI_D, p) * (double)q[g];
+            x[e] = odd ? cos(a) : sin(a);
+        }
+        else if (p == 6)
+            x[e] = odd ? (double)q[6 + g] : (double)q[3 + g];
     }
 }
 

@@ -3,8 +3,9 @@
 What each oracle function is checked against:
   oracle_bf16_round       torch's float32 -> bfloat16 conversion (round to
                           nearest even), ties included
-  oracle_guide_encode     sin^2 + cos^2 = 1 per pair and the double-angle
-                          identity between consecutive frequencies
+  oracle_guide_encode     sin^2 + cos^2 = 1 per pair, the double-angle
+                          identities between consecutive frequencies, the
+                          first frequency against numpy, 42 distinct slots
   oracle_guide_pdf        integral over the sphere = 1 (quadrature), the
                           textbook vMF value at the mean, uniform limit
   oracle_guide_sample     chi-square of a histogram against the pdf
@@ -39,21 +40,32 @@ def test_bf16_round_matches_torch():
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def _feature(g, p, odd):
+    """index e of the feature of lane group g, pair p, cosine if odd"""
+    return 16 * (p // 2) + 4 * g + 2 * (p % 2) + odd
+
+
 def test_encode_identities():
     rng = np.random.default_rng(1)
     for _ in range(20):
         q = np.concatenate([rng.uniform(0, 1, 3), rng.normal(size=6)]).astype(np.float32)
         x = ob.guide_encode(q)
-        s, c = x[0:36:2], x[1:36:2]
-        assert np.allclose(s * s + c * c, 1.0, atol=1e-12)
-        # pair (f, d) at e = 6 f + 2 d: sin(2a) = 2 sin(a) cos(a) for the next f
-        for f in range(5):
-            for d in range(3):
-                e, e2 = 6 * f + 2 * d, 6 * (f + 1) + 2 * d
-                assert abs(x[e2] - 2 * x[e] * x[e + 1]) < 1e-12
-        assert np.allclose(x[0:6:2], np.sin(np.pi * q[:3].astype(np.float64)))
-        assert np.allclose(x[36:42], q[3:9])
-        assert np.all(x[42:] == 0)
+        seen = np.zeros(64, bool)
+        for d in range(3):
+            s = np.array([x[_feature(d, f, 0)] for f in range(6)])
+            c = np.array([x[_feature(d, f, 1)] for f in range(6)])
+            assert np.allclose(s * s + c * c, 1.0, atol=1e-12)
+            # frequency f + 1 doubles the angle of frequency f
+            assert np.allclose(s[1:], 2 * s[:-1] * c[:-1], atol=1e-12)
+            assert np.allclose(c[1:], c[:-1] ** 2 - s[:-1] ** 2, atol=1e-12)
+            assert abs(s[0] - np.sin(np.pi * float(q[d]))) < 1e-12
+            assert abs(c[0] - np.cos(np.pi * float(q[d]))) < 1e-12
+            assert x[_feature(d, 6, 0)] == q[3 + d] and x[_feature(d, 6, 1)] == q[6 + d]
+            for f in range(7):
+                seen[_feature(d, f, 0)] = seen[_feature(d, f, 1)] = True
+        # 42 distinct features, the rest zero
+        assert seen.sum() == 42
+        assert np.all(x[~seen] == 0)
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])

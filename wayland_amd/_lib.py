@@ -80,7 +80,7 @@ class GuideConfig(C.Structure):
                 ("steps_per_batch", C.c_uint32), ("max_samples", C.c_uint32),
                 ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
                 ("epsilon", C.c_float), ("mix_alpha", C.c_float), ("max_weight", C.c_float),
-                ("guided_bounces", C.c_uint32), ("pad_", C.c_uint32)]
+                ("guided_bounces", C.c_uint32), ("guided_vertices", C.c_uint32)]
 
 
 class GuideInfo(C.Structure):

@@ -35,40 +35,45 @@ __device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c)
 }
 
 // ---- input encoding ------------------------------------------------------------
-// Query q[9]: position in [0,1]^3, normal, outgoing direction.  Feature e:
-// e < 36: sin (e even) / cos (e odd) of 2^(e/6) * pi * p[(e % 6) / 2];
-// 36..38 normal, 39..41 outgoing direction, 42..63 zero.
-// The lane's features come in pairs (e, e + 1) sharing one argument.  Which
-// coordinate a lane reads depends on its lane group, so q lives in LDS (a
-// private array indexed per lane would go to scratch memory).
+// Query q[9]: position in [0,1]^3, normal, outgoing direction.  Feature e,
+// with block B = e / 16, lane group g = (e % 16) / 4 and pair p = 2 B +
+// (e % 4) / 2 (the features a lane of group g holds, in its register order):
+//   g < 3, p < 6:  sin (e even) / cos (e odd) of 2^p * pi * q[g]
+//   g < 3, p = 6:  normal[g] (e even), outgoing direction[g] (e odd)
+//   otherwise 0.
+// So a lane encodes one axis: one sincospi, then five angle doublings
+// (sin 2a = 2 sin a cos a, cos 2a = (cos a - sin a)(cos a + sin a)), whose
+// error (about 2^5 FP32 ulps at f = 5) stays far below the BF16 operand's.
+// q lives in LDS: the axis a lane reads depends on its lane group, and a
+// private array indexed per lane would go to scratch memory.
 __device__ __forceinline__ void guide_encode(const float *q, int g, bf16x8 x[2])
 {
+    float v[16];
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
+    for (int k = 0; k < 16; ++k)
+        v[k] = 0.f;
+    if (g < 3)
+    {
+        float s, c;
+        sincospif(q[g], &s, &c); // exact argument reduction
 #pragma unroll
-        for (int j = 0; j < 8; j += 2)
+        for (int f = 0; f < 6; ++f)
         {
-            const int e = guide_phi(kb, g, j); // even; e + 1 is the cosine
-            float v0, v1;
-            if (e < 36)
-            {
-                const int f = e / 6, d = (e % 6) / 2;
-                // sin / cos of pi * 2^f * p; sincospi reduces its argument exactly
-                sincospif(ldexpf(q[d], f), &v0, &v1);
-            }
-            else if (e < 42)
-            {
-                v0 = q[e - 33]; // normal then outgoing direction
-                v1 = q[e - 32];
-            }
-            else
-            {
-                v0 = 0.f;
-                v1 = 0.f;
-            }
-            x[kb][j] = (__bf16)v0;
-            x[kb][j + 1] = (__bf16)v1;
+            v[2 * f] = s;
+            v[2 * f + 1] = c;
+            const float s2 = 2.0f * (s * c), c2 = (c - s) * (c + s);
+            s = s2;
+            c = c2;
         }
+        v[12] = q[3 + g];
+        v[13] = q[6 + g];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        x[0][j] = (__bf16)v[j];
+        x[1][j] = (__bf16)v[8 + j];
+    }
 }
 
 // ---- forward pass of one 16-query tile -------------------------------------

@@ -34,23 +34,6 @@ namespace wl
 
 constexpr int kPtBlock = 256;
 
-// ---- wave-level append: ballot + mbcnt prefix, one atomic per wave --------
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred)
-{
-    const uint64_t m = __ballot(pred);
-    if (m == 0)
-        return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t prefix = __builtin_amdgcn_mbcnt_hi(
-        (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader)
-        base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + prefix;
-}
-
 // ---- workgroup-level append to two queues -----------------------------------
 // Every wave of the workgroup calls this in the same loop pass.  Per wave a
 // ballot and mbcnt prefix; per workgroup one atomic per queue, issued by one
@@ -321,13 +304,60 @@ __device__ __forceinline__ void guide_query(const PtParams &P, V3 hitPos, V3 Ng,
     q[8] = -rayDir.z;
 }
 
+// Path state word thr.w: bits 0-15 the path's depth, bits 16-31 the number of
+// guided vertices it has passed.
+constexpr uint32_t kDepthMask = 0xFFFFu;
+
+// Whether the shade pass samples the guide at the vertex of queue item i: a
+// hit on a mesh that emits nothing, below the depth cap, whose material has
+// trans = 0 and clearcoat = 0, on a path that has passed fewer than gverts
+// guided vertices (gverts = 0: no limit).  The shade kernel decides from the
+// same values (its emitter, depth-cap and material branches, then gverts).
+// SEL 0: every hit (no path state: diagnostics), 1: a path's first vertex
+// (depth 0, none guided), 2: any later vertex (state read from thr).
+template <int SEL>
+__device__ __forceinline__ bool guide_wanted(const PtParams &P, const PtScene &S,
+                                             const PtBuffers &B, const uint32_t *qin,
+                                             uint32_t prim, uint32_t i)
+{
+    if (prim == WL_INVALID)
+        return false;
+    if (SEL == 0)
+        return true;
+    uint32_t depth = 0, gvis = 0;
+    if (SEL == 2)
+    {
+        const uint32_t raw = __float_as_uint(B.thr[qin[i]].w);
+        depth = raw & kDepthMask;
+        gvis = raw >> 16;
+    }
+    if (depth + 1 >= P.max_depth || (P.gverts != 0 && gvis >= P.gverts))
+        return false;
+    const uint32_t mesh = __float_as_uint(S.prims[12 * (size_t)prim + 9]);
+    const PtMesh M = S.meshes[mesh];
+    if (M.light != WL_INVALID)
+        return false;
+    const wl_disney_material &m = S.materials[M.material];
+    return m.trans == 0.0f && m.clearcoat == 0.0f;
+}
+
+// Guide inference for the active queue.  Each wave walks its 64-item chunks
+// of the queue, appends the items whose vertex the shade pass will guide to a
+// ring of item indices in LDS, and runs the network on full 16-query tiles
+// from the ring (the remainder at the end), so the MFMA work follows the
+// guided vertices, not the queue length.  Results go to gout at the item's
+// queue index; other items' slots are not written.
+template <int SEL>
 __global__ __launch_bounds__(kPtBlock)
 void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
 {
+    constexpr int kRing = 128; // holds < 16 left over + 64 appended
     __shared__ bf16x8 sA[kGuideFwdVecs];
     __shared__ float sb[kGuideBiases];
     __shared__ float sq[9 * kPtBlock];
+    __shared__ uint32_t sring[(kPtBlock / 64) * kRing];
     float *q = sq + 9 * threadIdx.x;
+    uint32_t *ring = sring + (threadIdx.x / 64) * kRing;
     const bf16x8 *fwd = reinterpret_cast<const bf16x8 *>(P.gfwd);
     for (int k = threadIdx.x; k < kGuideFwdVecs; k += kPtBlock)
         sA[k] = fwd[k];
@@ -336,46 +366,27 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
                         : P.gparams[guide_b_off(4) + (k - 256)];
     __syncthreads();
     const uint32_t count = B.counters[parity];
+    const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
     const int lane = threadIdx.x & 63, g = lane >> 4;
-    const uint32_t tiles = (count + 15) / 16;
     const uint32_t nwaves = gridDim.x * (kPtBlock / 64);
-    // software pipeline: the hit and ray of the wave's next tile are loaded
-    // while the current tile runs through the network
-    auto load_in = [&](uint32_t tile, float4 &h, float4 &a, float4 &b) {
-        const uint32_t i = tile * 16 + (lane & 15);
-        h = make_float4(0.f, __uint_as_float(WL_INVALID), 0.f, 0.f);
-        if (tile < tiles && i < count)
-        {
-            h = reinterpret_cast<const float4 *>(B.hits)[i];
-            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-            a = rp[0];
-            b = rp[1];
-        }
-    };
-    const uint32_t first = blockIdx.x * (kPtBlock / 64) + threadIdx.x / 64;
-    float4 nhit, nr0 = make_float4(0.f, 0.f, 0.f, 0.f), nr1 = nr0;
-    load_in(first, nhit, nr0, nr1);
-    for (uint32_t tile = first; tile < tiles; tile += nwaves)
-    {
-        const float4 hit = nhit, r0 = nr0, r1 = nr1;
-        load_in(tile + nwaves, nhit, nr0, nr1);
-        const uint32_t i = tile * 16 + (lane & 15);
-        const uint32_t prim = __float_as_uint(hit.y);
-        bool valid = i < count && prim != WL_INVALID;
-        float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0, w2 = w0;
-        if (valid)
-        {
-            const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
-            w0 = sr[0];
-            w1 = sr[1];
-            w2 = sr[2];
-        }
+    const uint32_t wave = blockIdx.x * (kPtBlock / 64) + threadIdx.x / 64;
+    // one tile: ring entries [head, head + n), n <= 16 (wave-uniform)
+    auto run_tile = [&](uint32_t head, uint32_t n) {
+        const uint32_t k = (uint32_t)(lane & 15);
+        const bool valid = k < n;
+        const uint32_t i = valid ? ring[(head + k) % kRing] : 0u;
 #pragma unroll
-        for (int k = 0; k < 9; ++k)
-            q[k] = 0.f;
+        for (int t = 0; t < 9; ++t)
+            q[t] = 0.f;
         if (valid)
         {
+            const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
+            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            const uint32_t prim = __float_as_uint(hit.y);
+            const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
+            const float4 w0 = sr[0], w1 = sr[1], w2 = sr[2];
             const V3 o = mk(r0.x, r0.y, r0.z), d = mk(r1.x, r1.y, r1.z);
             const V3 va = mk(w0.x, w0.y, w0.z), vb = mk(w0.w, w1.x, w1.y),
                      vc = mk(w1.z, w1.w, w2.x);
@@ -389,7 +400,34 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
         floatx4 o[2];
         guide_forward<false>(sA, sb, lane, x, o, nullptr);
         guide_decode_lanes(o, g, B.gout + kGuideDecoded * (size_t)i, valid);
+    };
+    uint32_t head = 0, tail = 0; // ring positions, wave-uniform
+    for (uint32_t base = wave * 64u; base < count; base += nwaves * 64u)
+    {
+        const uint32_t i = base + (uint32_t)lane;
+        bool want = false;
+        if (i < count)
+        {
+            const uint32_t prim = reinterpret_cast<const uint32_t *>(B.hits)[4 * (size_t)i + 1];
+            want = guide_wanted<SEL>(P, S, B, qin, prim, i);
+        }
+        const uint64_t m = __ballot(want);
+        if (want)
+            ring[(tail + lane_prefix(m)) % kRing] = i;
+        tail += (uint32_t)__popcll(m);
+        // the ring is read by other lanes of this wave: order the LDS
+        // accesses at wave scope
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        while (tail - head >= 16u)
+        {
+            run_tile(head, 16u);
+            head += 16u;
+        }
     }
+    if (tail != head)
+        run_tile(head, tail - head);
 }
 
 // ---- shade ------------------------------------------------------------------
@@ -458,7 +496,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     s_dir = f4(d, __uint_as_float(sd)); // w: seed
                 }
                 s_thr = f4(mk(1, 1, 1), __uint_as_float(0u)); // w: depth
-                s_ltt = f4(mk(1, 1, 1), 0.0f);
+                s_ltt = f4(mk(1, 1, 1), 1.0f); // w: miss weight
             }
             else
             {
@@ -472,8 +510,10 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             V3 rayDir = v3of(s_dir);
             uint32_t seed = __float_as_uint(s_dir.w);
             V3 throughput = v3of(s_thr);
-            uint32_t depth = __float_as_uint(s_thr.w);
+            uint32_t depth = __float_as_uint(s_thr.w) & kDepthMask;
+            uint32_t gvis = __float_as_uint(s_thr.w) >> 16; // guided vertices passed
             V3 lastTraceTerm = v3of(s_ltt);
+            float missw = 1.0f; // miss weight of the ray this vertex samples
             // radiance changes only at a miss or an emitter hit: read and
             // written back only then (or when guide training needs it)
             const bool gtrain = GTRAIN && (P.gmode & 2u);
@@ -492,10 +532,17 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             const V3 rad_in = radiance;
             if (prim == WL_INVALID)
             {
-                // __miss__radiance: radiance += throughput * bg
+                // __miss__radiance: radiance += throughput * bg.  The
+                // reference leaves the last vertex's weight f |cos| / pdf
+                // out of this term, so its expectation is throughput * bg
+                // times the probability that the last vertex's sampler
+                // sends the ray to the background.  s_ltt.w keeps that
+                // expectation when a guided vertex sampled the ray: the
+                // reference sampler's density over the mixed density
+                // there, and exactly 1 after every other vertex.
                 if (!gtrain)
                     radiance = v3of(B.rad[path]);
-                radiance = radiance + throughput * ld3(P.bg);
+                radiance = radiance + throughput * ld3(P.bg) * s_ltt.w;
                 rad_dirty = true;
                 done = true;
             }
@@ -574,8 +621,11 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                         if (dot(Ns, rayDir) > 0)
                             Ns = -Ns;
                     }
-                    const bool guided =
-                        GSAMPLE && mat.trans == 0.0f && mat.clearcoat == 0.0f && (P.gmode & 1u);
+                    const bool guided = GSAMPLE && mat.trans == 0.0f &&
+                                        mat.clearcoat == 0.0f && (P.gmode & 1u) &&
+                                        (P.gverts == 0 || gvis < P.gverts);
+                    if (GSAMPLE && guided)
+                        gvis += 1;
                     GuideMix gm;
                     bool took_guide = false;
                     V3 nd;
@@ -600,9 +650,18 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     float pdf = pdf_disney(mat, Ns, -rayDir, nd);
                     if (GSAMPLE && guided && !(pdf < 0.0f || __builtin_isnan(pdf)))
                     {
+                        // the reference sampler's own density: its pdf less
+                        // the diffuse term below the surface, where the
+                        // sampler draws nothing (trans = 0, clearcoat = 0)
+                        const float cl = dot(nd, Ns);
+                        float sdens = pdf;
+                        if (cl < 0.0f)
+                            sdens -= 0.5f * (1.0f - mat.metallic) * (-cl * (1.0f / kPi));
                         pdf = P.galpha * guide_pdf(gm, nd) + (1.0f - P.galpha) * pdf;
                         if (!(pdf > 0.0f))
                             pdf = -1.0f; // a zero mixed density ends the path
+                        else
+                            missw = fmaxf(sdens, 0.0f) / pdf;
                     }
                     if (pdf < 0.0f || __builtin_isnan(pdf))
                     {
@@ -745,8 +804,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 cont = true;
                 B.pos[path] = f4(rayPos, lastPdf);
                 B.dir[path] = f4(rayDir, __uint_as_float(seed));
-                B.thr[path] = f4(throughput, __uint_as_float(depth));
-                B.ltt[path] = f4(lastTraceTerm, 0.0f);
+                B.thr[path] = f4(throughput, __uint_as_float(depth | (gvis << 16)));
+                B.ltt[path] = f4(lastTraceTerm, missw);
                 next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
                                    rayDir.x, rayDir.y, rayDir.z, 1e30f };
             }
@@ -913,37 +972,79 @@ __global__ __launch_bounds__(kPtBlock) void k_accumulate_multi(PtMultiLaunch L, 
 // the path gathered through that direction divided by the throughput that
 // reached the vertex (channel means).  Zero-weight records carry no gradient
 // and are dropped.
+// Each thread takes kCollectPer paths per pass (coalesced: path base + k
+// kPtBlock + thread) and the workgroup claims the training slots of the whole
+// pass with one atomic: one per 2048 paths, where one per wave serialised
+// the kernel on the counter.
+constexpr uint32_t kCollectPer = 8;
+
 __global__ __launch_bounds__(kPtBlock) void k_guide_collect(PtParams P, PtBuffers B)
 {
+    __shared__ uint32_t s_cnt[kPtBlock / 64];
+    __shared__ uint32_t s_base;
     const uint32_t npaths = P.npix * P.nsamples;
-    const uint32_t nwaves_all = (npaths + 63) / 64;
-    for (uint32_t base = blockIdx.x * kPtBlock; base < nwaves_all * 64u;
-         base += gridDim.x * kPtBlock)
+    const uint32_t wave = threadIdx.x / 64u, lane = __lane_id();
+    constexpr uint32_t span = kPtBlock * kCollectPer;
+    // block-uniform loop: every thread runs the same passes (barriers)
+    for (uint32_t base = blockIdx.x * span; base < npaths; base += gridDim.x * span)
     {
-        const uint32_t p = base + threadIdx.x;
-        bool keep = false;
-        float w = 0.f;
-        if (p < npaths)
+        float w[kCollectPer];
+        uint64_t m[kCollectPer];
+        uint32_t wtotal = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kCollectPer; ++k)
         {
-            const float4 ga = B.gacc[p];
-            if (ga.w != 0.0f)
+            const uint32_t p = base + k * kPtBlock + threadIdx.x;
+            bool keep = false;
+            w[k] = 0.f;
+            if (p < npaths)
             {
-                const float thr = B.grec[16 * (size_t)p + 13];
-                w = (ga.x + ga.y + ga.z) * (1.0f / 3.0f) / fmaxf(thr, 1e-30f);
-                keep = w > 0.0f; // false for NaN
-                w = fminf(w, B.gwmax);
+                const float4 ga = B.gacc[p];
+                if (ga.w != 0.0f)
+                {
+                    const float thr = B.grec[16 * (size_t)p + 13];
+                    const float wk = (ga.x + ga.y + ga.z) * (1.0f / 3.0f) / fmaxf(thr, 1e-30f);
+                    keep = wk > 0.0f; // false for NaN
+                    w[k] = fminf(wk, B.gwmax);
+                }
             }
+            m[k] = __ballot(keep);
+            wtotal += (uint32_t)__popcll(m[k]);
         }
-        const uint32_t slot = wave_append(B.gcount, keep);
-        if (keep && slot < B.gtrain_cap)
+        if (lane == 0)
+            s_cnt[wave] = wtotal;
+        __syncthreads();
+        if (threadIdx.x == 0)
         {
-            const float4 *src = reinterpret_cast<const float4 *>(B.grec + 16 * (size_t)p);
-            float4 *dst = reinterpret_cast<float4 *>(B.gtrain + 16 * (size_t)slot);
-            dst[0] = src[0];
-            dst[1] = src[1];
-            dst[2] = src[2];
-            dst[3] = make_float4(w, 0.0f, 0.0f, 0.0f);
+            uint32_t t = 0;
+            for (uint32_t v = 0; v < kPtBlock / 64; ++v)
+                t += s_cnt[v];
+            s_base = t ? atomicAdd(B.gcount, t) : 0u;
         }
+        __syncthreads();
+        uint32_t o = s_base;
+        for (uint32_t v = 0; v < wave; ++v)
+            o += s_cnt[v];
+#pragma unroll
+        for (uint32_t k = 0; k < kCollectPer; ++k)
+        {
+            if ((m[k] >> lane) & 1u)
+            {
+                const uint32_t slot = o + lane_prefix(m[k]);
+                const uint32_t p = base + k * kPtBlock + threadIdx.x;
+                if (slot < B.gtrain_cap)
+                {
+                    const float4 *src = reinterpret_cast<const float4 *>(B.grec + 16 * (size_t)p);
+                    float4 *dst = reinterpret_cast<float4 *>(B.gtrain + 16 * (size_t)slot);
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                    dst[2] = src[2];
+                    dst[3] = make_float4(w[k], 0.0f, 0.0f, 0.0f);
+                }
+            }
+            o += (uint32_t)__popcll(m[k]);
+        }
+        __syncthreads(); // s_cnt and s_base are rewritten by the next pass
     }
 }
 
@@ -1084,8 +1185,12 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
         const bool gtrain = (P.gmode & 2u) != 0;
         if (gsample)
         {
-            hipLaunchKernelGGL(k_guide_infer_pt, dim3(g_guide), dim3(kPtBlock), 0, st, P, S, B,
-                               parity);
+            if (b == 0)
+                hipLaunchKernelGGL(k_guide_infer_pt<1>, dim3(g_guide), dim3(kPtBlock), 0, st, P,
+                                   S, B, parity);
+            else
+                hipLaunchKernelGGL(k_guide_infer_pt<2>, dim3(g_guide), dim3(kPtBlock), 0, st, P,
+                                   S, B, parity);
             if (gtrain)
                 hipLaunchKernelGGL((k_shade<true, true, false>), dim3(g_shade), dim3(kPtBlock), 0,
                                    st, P, S, B, parity, nullptr, b == 0 ? 1 : 0);
@@ -1204,15 +1309,16 @@ hipError_t pt_guide_infer_queue(const PtParams &P, const PtScene &S, const PtBuf
                                 uint32_t n, int num_cus, hipStream_t st)
 {
     const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, (uint32_t)num_cus * 4));
-    hipLaunchKernelGGL(k_guide_infer_pt, dim3(g), dim3(kPtBlock), 0, st, P, S, B, 0);
+    hipLaunchKernelGGL(k_guide_infer_pt<0>, dim3(g), dim3(kPtBlock), 0, st, P, S, B, 0);
     return hipGetLastError();
 }
 
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus, hipStream_t st)
 {
     const uint32_t npaths = P.npix * P.nsamples;
+    const uint32_t span = kPtBlock * kCollectPer;
     const uint32_t g =
-        std::max<uint32_t>(1, std::min<uint32_t>((npaths + kPtBlock - 1) / kPtBlock, (uint32_t)num_cus * 8));
+        std::max<uint32_t>(1, std::min<uint32_t>((npaths + span - 1) / span, (uint32_t)num_cus * 8));
     hipLaunchKernelGGL(k_guide_collect, dim3(g), dim3(kPtBlock), 0, st, P, B);
     return hipGetLastError();
 }

@@ -73,6 +73,7 @@ struct PtParams
     uint32_t sort_bins;        // material-sorted shade queue: bins (0 = off)
     uint32_t gmode;
     uint32_t gbounces;         // guided sampling on bounces [0, gbounces), 0 = all
+    uint32_t gverts;           // guided vertices per path at most (0 = no limit)
     float galpha;              // probability of sampling the guide
     const void *gfwd;          // packed BF16 forward operands
     const float *gparams;      // FP32 master parameters (biases)
@@ -84,8 +85,8 @@ struct PtBuffers
 {
     float4 *pos;      // ray origin, w = lastPdf
     float4 *dir;      // ray direction, w = seed bits
-    float4 *thr;      // throughput, w = depth bits
-    float4 *ltt;      // lastTraceTerm
+    float4 *thr;      // throughput, w = depth (bits 0-15), guided vertices (16-31)
+    float4 *ltt;      // lastTraceTerm, w = weight of a miss of the next ray
     float4 *rad;      // radiance
     wl_ray *rays0, *rays1;
     uint32_t *qpath0, *qpath1;
@@ -185,12 +186,13 @@ hipError_t pt_clock_stamp(unsigned long long *dst, hipStream_t st);
 // tri0 unchanged).
 hipError_t pt_arena_copy_nodes(WlBvh8Node *dst, const WlBvh8Node *src, uint32_t n,
                                uint32_t node0, uint32_t tri0, hipStream_t st);
-// Moves the finished batch's training records with a positive weight into
-// the training buffer.
 // The path tracer's guide inference (k_guide_infer_pt) over queue 0 of B:
-// n = B.counters[0] entries of rays0 / hits, decoded mixtures to B.gout.
+// n = B.counters[0] entries of rays0 / hits, decoded mixtures of every hit
+// to B.gout (misses are not written).
 hipError_t pt_guide_infer_queue(const PtParams &P, const PtScene &S, const PtBuffers &B,
                                 uint32_t n, int num_cus, hipStream_t st);
+// Moves the finished batch's training records with a positive weight into
+// the training buffer.
 hipError_t pt_guide_collect(const PtParams &P, const PtBuffers &B, int num_cus,
                             hipStream_t st);
 hipError_t pt_accumulate(const PtParams &P, const PtBuffers &B, float *film,

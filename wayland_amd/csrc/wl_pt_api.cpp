@@ -231,6 +231,7 @@ wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
         P.gmode = s->guide_mode;
         P.galpha = s->guide->cfg.mix_alpha;
         P.gbounces = s->guide->cfg.guided_bounces;
+        P.gverts = s->guide->cfg.guided_vertices;
         P.gfwd = s->guide->dev.fwd;
         P.gparams = s->guide->dev.params;
         std::memcpy(P.gbox_lo, s->box_lo, sizeof P.gbox_lo);
