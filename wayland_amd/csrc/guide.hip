@@ -70,12 +70,6 @@ __global__ void k_guide_pack(const float *__restrict__ P, bf16x8 *__restrict__ f
     }
 }
 
-// Biases in the order guide_forward reads them: 64 per hidden layer, then 32.
-__device__ __forceinline__ float bias_at(const float *P, int i)
-{
-    return i < 256 ? P[guide_b_off(i / 64) + i % 64] : P[guide_b_off(4) + (i - 256)];
-}
-
 // ---- inference on query records -------------------------------------------------
 __global__ __launch_bounds__(kGuideBlock)
 void k_guide_infer(const bf16x8 *__restrict__ fwd,
@@ -87,10 +81,8 @@ void k_guide_infer(const bf16x8 *__restrict__ fwd,
     __shared__ float sb[kGuideBiases];
     __shared__ float sq[9 * kGuideBlock];
     float *q = sq + 9 * threadIdx.x;
-    for (int i = threadIdx.x; i < kGuideFwdVecs; i += kGuideBlock)
-        sA[i] = fwd[i];
-    for (int i = threadIdx.x; i < kGuideBiases; i += kGuideBlock)
-        sb[i] = bias_at(P, i);
+    guide_stage<kGuideBlock, kGuideFwdVecs>(sA, fwd);
+    guide_stage_biases<kGuideBlock>(sb, P);
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const uint32_t tiles = (n + 15) / 16;
@@ -224,8 +216,8 @@ __device__ __forceinline__ float guide_loss_grad(const floatx4 out[2], V3 om, fl
         pi[t] = (t == 0 ? e0 : e1) / se;
         kap[t] = guide_softplus(out[t][1]);
         float st, ct, sp, cp;
-        sincosf(out[t][2], &st, &ct);
-        sincosf(out[t][3], &sp, &cp);
+        guide_sincos(out[t][2], &st, &ct);
+        guide_sincos(out[t][3], &sp, &cp);
         const V3 mu = mk(st * cp, st * sp, ct);
         cth[t] = dot(mu, om);
         wdth[t] = om.x * (ct * cp) + om.y * (ct * sp) - om.z * st;
@@ -260,12 +252,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_fwdbwd(GuideDev G,
     float *q = sq + 9 * threadIdx.x;
     const bf16x8 *fwd = reinterpret_cast<const bf16x8 *>(G.fwd);
     const bf16x8 *bwd = reinterpret_cast<const bf16x8 *>(G.bwd);
-    for (int i = threadIdx.x; i < kGuideFwdVecs; i += kGuideBlock)
-        sA[i] = fwd[i];
-    for (int i = threadIdx.x; i < kGuideBwdVecs; i += kGuideBlock)
-        sB[i] = bwd[i];
-    for (int i = threadIdx.x; i < kGuideBiases; i += kGuideBlock)
-        sb[i] = bias_at(G.params, i);
+    guide_stage<kGuideBlock, kGuideFwdVecs>(sA, fwd);
+    guide_stage<kGuideBlock, kGuideBwdVecs>(sB, bwd);
+    guide_stage_biases<kGuideBlock>(sb, G.params);
     __syncthreads();
     uint32_t offset, size;
     batch_range(*n_dev, n_cap, step, batch, offset, size);
@@ -379,7 +368,33 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_dw(GuideDev G,
                       (size_t)(im * 16 + (lane & 15)) * cap;
     floatx4 acc = { 0.f, 0.f, 0.f, 0.f };
     float bs = 0.f;
-    for (uint32_t st = st0; st < st1; ++st)
+    // four 32-sample steps per round, their eight loads issued before the
+    // first MFMA waits on them
+    constexpr uint32_t kRound = 4;
+    uint32_t st = st0;
+    for (; st + kRound <= st1; st += kRound)
+    {
+        bf16x8 a[kRound], b[kRound];
+#pragma unroll
+        for (uint32_t k = 0; k < kRound; ++k)
+        {
+            const size_t s0 = (size_t)(st + k) * 32 + 8 * g;
+            a[k] = *reinterpret_cast<const bf16x8 *>(D + s0);
+            b[k] = *reinterpret_cast<const bf16x8 *>(X + s0);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kRound; ++k)
+        {
+            acc = mfma16(a[k], b[k], acc);
+            if (im == 0)
+            {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    bs += (float)a[k][j];
+            }
+        }
+    }
+    for (; st < st1; ++st)
     {
         const size_t s0 = (size_t)st * 32 + 8 * g;
         const bf16x8 a = *reinterpret_cast<const bf16x8 *>(D + s0);

@@ -304,9 +304,12 @@ __device__ __forceinline__ void guide_query(const PtParams &P, V3 hitPos, V3 Ng,
     q[8] = -rayDir.z;
 }
 
-// Path state word thr.w: bits 0-15 the path's depth, bits 16-31 the number of
-// guided vertices it has passed.
+// Path state word thr.w: bits 0-15 the path's depth, bits 16-30 the number of
+// guided vertices it has passed, bit 31 set once its guide training record
+// is taken.
 constexpr uint32_t kDepthMask = 0xFFFFu;
+constexpr uint32_t kGvisMask = 0x7FFFu;
+constexpr uint32_t kGrecBit = 1u << 31;
 
 // Whether the shade pass samples the guide at the vertex of queue item i: a
 // hit on a mesh that emits nothing, below the depth cap, whose material has
@@ -329,7 +332,7 @@ __device__ __forceinline__ bool guide_wanted(const PtParams &P, const PtScene &S
     {
         const uint32_t raw = __float_as_uint(B.thr[qin[i]].w);
         depth = raw & kDepthMask;
-        gvis = raw >> 16;
+        gvis = (raw >> 16) & kGvisMask;
     }
     if (depth + 1 >= P.max_depth || (P.gverts != 0 && gvis >= P.gverts))
         return false;
@@ -359,11 +362,8 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
     float *q = sq + 9 * threadIdx.x;
     uint32_t *ring = sring + (threadIdx.x / 64) * kRing;
     const bf16x8 *fwd = reinterpret_cast<const bf16x8 *>(P.gfwd);
-    for (int k = threadIdx.x; k < kGuideFwdVecs; k += kPtBlock)
-        sA[k] = fwd[k];
-    for (int k = threadIdx.x; k < kGuideBiases; k += kPtBlock)
-        sb[k] = k < 256 ? P.gparams[guide_b_off(k / 64) + k % 64]
-                        : P.gparams[guide_b_off(4) + (k - 256)];
+    guide_stage<kPtBlock, kGuideFwdVecs>(sA, fwd);
+    guide_stage_biases<kPtBlock>(sb, P.gparams);
     __syncthreads();
     const uint32_t count = B.counters[parity];
     const uint32_t *qin = parity ? B.qpath1 : B.qpath0;
@@ -510,26 +510,22 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             V3 rayDir = v3of(s_dir);
             uint32_t seed = __float_as_uint(s_dir.w);
             V3 throughput = v3of(s_thr);
-            uint32_t depth = __float_as_uint(s_thr.w) & kDepthMask;
-            uint32_t gvis = __float_as_uint(s_thr.w) >> 16; // guided vertices passed
+            const uint32_t sw = __float_as_uint(s_thr.w);
+            uint32_t depth = sw & kDepthMask;
+            uint32_t gvis = (sw >> 16) & kGvisMask; // guided vertices passed
             V3 lastTraceTerm = v3of(s_ltt);
             float missw = 1.0f; // miss weight of the ray this vertex samples
             // radiance changes only at a miss or an emitter hit: read and
-            // written back only then (or when guide training needs it)
+            // written back only then
             const bool gtrain = GTRAIN && (P.gmode & 2u);
-            V3 radiance = mk(0, 0, 0);
+            V3 radiance = mk(0, 0, 0), rad_in = mk(0, 0, 0);
             bool rad_dirty = false;
-            if (gtrain)
-                radiance = v3of(B.rad[path]);
             const uint32_t prim = __float_as_uint(hit.y);
             bool done = false;
-            // guiding: training record state of the path
-            float4 gacc = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gtrain)
-                gacc = B.gacc[path];
-            const bool grec_done = gacc.w != 0.f; // first guided vertex recorded
+            // guiding: the path's training record is taken (its first guided
+            // vertex); the radiance added after it goes to gacc
+            const bool grec_done = gtrain && (sw & kGrecBit) != 0;
             bool grec_now = false;
-            const V3 rad_in = radiance;
             if (prim == WL_INVALID)
             {
                 // __miss__radiance: radiance += throughput * bg.  The
@@ -540,8 +536,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 // expectation when a guided vertex sampled the ray: the
                 // reference sampler's density over the mixed density
                 // there, and exactly 1 after every other vertex.
-                if (!gtrain)
-                    radiance = v3of(B.rad[path]);
+                radiance = v3of(B.rad[path]);
+                rad_in = radiance;
                 radiance = radiance + throughput * ld3(P.bg) * s_ltt.w;
                 rad_dirty = true;
                 done = true;
@@ -586,8 +582,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                             neePdf = pdf * dist * dist / fabsf(dot(Ns, rayDir));
                         }
                         const float uptPdf = lastPdf;
-                        if (!gtrain)
-                            radiance = v3of(B.rad[path]);
+                        radiance = v3of(B.rad[path]);
+                        rad_in = radiance;
                         rad_dirty = true;
                         radiance = radiance + throughput * ld3(lt.L) * lastTraceTerm *
                                                   (uptPdf / (uptPdf + neePdf));
@@ -787,16 +783,16 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (gtrain)
             {
                 // radiance reaching the path after its first guided vertex
-                if (grec_done)
+                if (grec_done && rad_dirty)
                 {
+                    float4 gacc = B.gacc[path];
                     gacc.x += radiance.x - rad_in.x;
                     gacc.y += radiance.y - rad_in.y;
                     gacc.z += radiance.z - rad_in.z;
-                }
-                if (grec_now)
-                    gacc.w = 1.0f;
-                if (grec_done || grec_now)
                     B.gacc[path] = gacc;
+                }
+                if (grec_now) // nothing gathered yet; w marks the record
+                    B.gacc[path] = make_float4(0.f, 0.f, 0.f, 1.0f);
                 shadow_gacc = grec_done;
             }
             if (!done)
@@ -804,7 +800,9 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 cont = true;
                 B.pos[path] = f4(rayPos, lastPdf);
                 B.dir[path] = f4(rayDir, __uint_as_float(seed));
-                B.thr[path] = f4(throughput, __uint_as_float(depth | (gvis << 16)));
+                B.thr[path] = f4(throughput,
+                                 __uint_as_float(depth | (gvis << 16) |
+                                                 (grec_done || grec_now ? kGrecBit : 0u)));
                 B.ltt[path] = f4(lastTraceTerm, missw);
                 next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
                                    rayDir.x, rayDir.y, rayDir.z, 1e30f };
