@@ -14,12 +14,13 @@
 // the 64 lanes of a wave touch 64 consecutive words.
 //
 // Two step shapes share the node and triangle code.  trav_step visits a node
-// and then tests all of its triangles (any-hit, which stops at the first hit,
-// and the instrumented kernel).  The closest-hit kernels step with
-// trav_step_one instead: a lane either visits one node or
-// tests one pending triangle per step, so lanes with several triangles do not
-// hold the whole wave in a triangle loop.  Both visit the same nodes and test
-// the same triangles in the same order, so the results are the same.
+// and then tests all of its triangles (the instrumented kernel and the
+// one-ray-to-completion loop).  The persistent kernels, closest and any-hit,
+// step with trav_step_one instead: a lane tests one pending triangle per
+// step, and visits its next node once none is pending (in the same step as
+// its last triangle), so lanes with several triangles do not hold the whole
+// wave in a triangle loop.  Both visit the same nodes and test the same
+// triangles in the same order, so the results are the same.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -424,10 +425,13 @@ WL_HD bool trav_step(const WlBvh8Node *__restrict__ nodes,
     return false;
 }
 
-// Closest-hit step: one pending triangle, or else one node (the next from
-// the group or the stack, then its box test).  Returns true when the ray is
-// finished.
-template <int BLOCK, bool AUX = false>
+// Persistent-kernel step: one pending triangle, then, once no triangle is
+// pending, one node (the next from the group or the stack, then its box
+// test).  A lane that tests its node's last triangle goes on to the next node
+// in the same step: the order of tests is that of doing the two one after the
+// other, and the wave's node branch runs with more of its lanes.  Returns
+// true when the ray is finished (no node left, or with ANY its first hit).
+template <int BLOCK, bool AUX = false, bool ANY = false>
 WL_HD bool trav_step_one(const WlBvh8Node *__restrict__ nodes,
                          const WlTriRecord *__restrict__ tris, const TravRay &r,
                          TravState &st, uint32_t *lds_stack_lane,
@@ -438,8 +442,10 @@ WL_HD bool trav_step_one(const WlBvh8Node *__restrict__ nodes,
         const uint32_t i = __builtin_ctz(st.trimask);
         st.trimask &= st.trimask - 1u;
         WL_TRAV_CHECK((size_t)st.tri_base + i < wl_trav_tris_bound, "triangle index");
-        tri_visit<false, AUX>(tris, st.tri_base, i, r, st);
-        return false;
+        if (tri_visit<ANY, AUX>(tris, st.tri_base, i, r, st) && ANY)
+            return true;
+        if (st.trimask)
+            return false;
     }
     if (!st.visit)
     {

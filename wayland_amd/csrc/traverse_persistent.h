@@ -3,11 +3,10 @@
 // Incoherent rays finish after very different numbers of steps, so a wave
 // that runs one ray per lane to completion idles most of its lanes near the
 // end.  Here a fixed grid of waves loops: every lane advances its ray by one
-// step (any-hit: a node and its triangles, trav_step; closest hit: one node
-// or one triangle, trav_step_one); once at least REFILL lanes of a
-// wave are idle, the idle lanes start the next rays of the wave's own ray
-// indices (after Aila & Laine, "Understanding the Efficiency of Ray Traversal
-// on GPUs", HPG 2009).
+// step (trav_step_one: one pending triangle, then the next node once none is
+// pending); once at least REFILL lanes of a wave are idle, the idle lanes
+// start the next rays of the wave's own ray indices (after Aila & Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009).
 //
 // Each wave owns ceil(n / waves) rays (one range, or interleaved chunks) and
 // refills from a wave-uniform register, so no atomics are needed.  The
@@ -107,7 +106,6 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
     trav_init(st, r, false);
     uint32_t rid = 0;
     bool alive = false;
-    uint32_t nn = 0, nt = 0;
     for (;;)
     {
         const uint64_t idle = __ballot(!alive);
@@ -148,11 +146,7 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
         bool fin = false;
         if (alive)
         {
-            if constexpr (ANY)
-                fin = trav_step<ANY, false, BLOCK, AUX>(nodes, tris, r, st, stack_lane, nn, nt,
-                                                        top, ntop);
-            else
-                fin = trav_step_one<BLOCK, AUX>(nodes, tris, r, st, stack_lane, top, ntop);
+            fin = trav_step_one<BLOCK, AUX, ANY>(nodes, tris, r, st, stack_lane, top, ntop);
         }
         if (fin)
         {
