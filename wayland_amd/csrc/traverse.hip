@@ -52,7 +52,7 @@ void k_trace_closest_p(
     __shared__ uint4 top[8 * (top_nodes<STACK>() ? top_nodes<STACK>() : 1)];
     if (top_nodes<STACK>())
         load_top(top, nodes, ntop);
-    trav_persistent<false, kTraceBlock, false, false, true, 4, 24>(
+    trav_persistent<false, kTraceBlock, false, false, true, 4>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); },

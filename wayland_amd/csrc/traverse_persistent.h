@@ -41,9 +41,9 @@ __device__ __forceinline__ void store_hit_nt(wl_hit *__restrict__ hits, uint32_t
 }
 
 // Default refill threshold: a wave refills once at least this many of its
-// lanes are idle (REFILL below overrides it per kernel; measured best: 24 for
-// 2^23 incoherent closest-hit rays, 20 for the path tracer's queues, 16 for
-// raw any-hit).
+// lanes are idle (REFILL below overrides it per kernel; measured best: 16 for
+// 2^23 incoherent closest-hit and any-hit rays, 8 to 16 within noise for
+// closest hit; 16 to 28 equal for the path tracer's queues, which use 20).
 constexpr int kRefillLanes = 16;
 
 __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
