@@ -51,7 +51,7 @@ def parse():
     p.add_argument("--guide-spp", type=int, default=64,
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
-    p.add_argument("--guide-runs", type=int, default=30)
+    p.add_argument("--guide-runs", type=int, default=60)
     p.add_argument("--serve-requests", type=int, default=64,
                    help="batched-serving requests (0 = skip)")
     p.add_argument("--scene-file-frames", type=int, default=64,

@@ -13,7 +13,9 @@ Pinning status per oracle function
   disney eval (reflection) metallic = 1 -> Walter 2007 GGX microfacet BRDF;
                            dielectric -> Burley 2012 diffuse + GGX; symmetry
   disney pdf / sample      pdf integrates to <= 1 (~1 for rough opaque);
-                           sampler histogram vs pdf (opaque, clearcoat 0)
+                           sampler histogram vs pdf (opaque, clearcoat 0);
+                           below the surface the sampler's density is the
+                           pdf less its diffuse term (whole-sphere histogram)
   disney eval (transmit),  pinned in test_oracle_pins.py: smooth-surface
   sample/pdf (refraction,  limit against Snell + Fresnel; reversal identity;
   TIR), MaterialMath,      microfacet normals of refracted samples vs GGX;
@@ -320,6 +322,67 @@ def test_sampler_histogram_matches_pdf():
     dof = big.sum() - 1
     assert chi2 < dof + 6 * math.sqrt(2 * dof)  # 6-sigma bound
     assert abs(obs.sum() - exp.sum()) / N < 0.03
+
+
+@pytest.mark.parametrize("metallic,rough", [(0.3, 0.6), (1.0, 0.8), (0.0, 1.0)])
+def test_sampler_density_below_surface_is_pdf_less_diffuse(metallic, rough):
+    """The guided shade pass weights a background miss by the reference
+    sampler's own density over the mixed one, taking that density as
+    PdfDisneyBSDF less its diffuse term 0.5 (1 - metallic) |L.N| / pi below
+    the surface (Pdf.h:98 keeps |L.N| there, while SampleCosineHemisphere,
+    Sample.h:43-53, draws nothing below).  Checked by histogram of the
+    opaque sampler's directions over the whole sphere at a grazing view
+    (rough lobes send many reflections below the surface): the pdf less that
+    term matches on both hemispheres, the unmodified pdf does not below."""
+    rng = np.random.default_rng(11)
+    n = np.array([0, 0, 1.0])
+    th = np.radians(75.0)
+    v = np.array([np.sin(th), 0.0, np.cos(th)])
+    mat = scenes.disney_material(color=(0.5, 0.5, 0.5), metallic=metallic, roughness=rough)
+    N = 200000
+    S = ob.disney_sample_batch(mat, n, n, v, rng.integers(0, 2 ** 32, N, dtype=np.uint64))
+    S /= np.linalg.norm(S, axis=1, keepdims=True)
+    # the GGX reflection density has an integrable singularity at L = -V
+    # (1 / |L.H| as V.H -> 0), which a midpoint rule cannot integrate: that
+    # cone is left out of both the samples and the quadrature
+    keep_dir = lambda d: d @ -v < np.cos(np.radians(25.0))
+    S = S[keep_dir(S)]
+    nb_c, nb_p = 10, 12
+
+    def bins(d):
+        c = np.clip(((d[:, 2] + 1) / 2 * nb_c).astype(int), 0, nb_c - 1)
+        p = np.clip(((np.arctan2(d[:, 1], d[:, 0]) + np.pi) / (2 * np.pi) * nb_p).astype(int),
+                    0, nb_p - 1)
+        return c * nb_p + p
+
+    obs = np.bincount(bins(S), minlength=nb_c * nb_p).astype(float)
+    # midpoint rule on a grid uniform in (z, phi), which is area-preserving
+    # (dA = dz dphi): 32 x 32 points per bin
+    k = 32
+    z = -1 + (np.arange(nb_c * k) + 0.5) * (2.0 / (nb_c * k))
+    ph = -np.pi + (np.arange(nb_p * k) + 0.5) * (2 * np.pi / (nb_p * k))
+    Z, P = np.meshgrid(z, ph, indexing="ij")
+    r = np.sqrt(1 - Z * Z)
+    U = np.stack([r * np.cos(P), r * np.sin(P), Z], -1).reshape(-1, 3)
+    U = U[keep_dir(U)]
+    pdf = ob.disney_pdf_batch(mat, n, n, v, U)
+    diff = 0.5 * (1.0 - metallic) * np.abs(U[:, 2]) / np.pi
+    dens = pdf - np.where(U[:, 2] < 0, diff, 0.0)
+    dA = (2.0 / (nb_c * k)) * (2 * np.pi / (nb_p * k))
+    exp = np.bincount(bins(U), weights=dens * dA, minlength=nb_c * nb_p) * N
+    exp_raw = np.bincount(bins(U), weights=pdf * dA, minlength=nb_c * nb_p) * N
+    below = np.arange(nb_c * nb_p) < (nb_c // 2) * nb_p
+    big = exp > 80
+    chi2 = ((obs[big] - exp[big]) ** 2 / exp[big]).sum()
+    dof = big.sum() - 1
+    assert chi2 < dof + 6 * math.sqrt(2 * dof)
+    # the sampler does send directions below the surface here, and their
+    # share is the corrected density's, not the raw pdf's
+    f_obs = obs[below].sum() / N
+    assert f_obs > 0.01
+    assert abs(f_obs - exp[below].sum() / N) < 0.004
+    if metallic < 1.0:
+        assert exp_raw[below].sum() / N - f_obs > 0.02
 
 
 def test_transmission_lobe_positive_finite():
