@@ -116,3 +116,33 @@ def test_guided_glossy_caustic_gain(gpu_ctx):
     # the firefly-robust error: guided below plain at the same sample count
     assert _relmse(img_g, ref, 0.001) < 0.7 * _relmse(img_p, ref, 0.001)
     guide.close()
+
+
+def test_guided_vertex_with_infinite_bsdf_pdf_stays_finite(gpu_ctx):
+    """Frame (19 << 20) + 62 of the glossy caustic at 1024 x 1024 draws, at
+    pixel 895139 (x 163, y 874), a GGX reflection on the metal floor that
+    folds back onto -V: the reference pdf there is +inf (1 / |L.H|) and the
+    direction lies below the surface.  At a guided vertex the miss weight of
+    that ray takes its limit 1 / (1 - alpha) instead of inf / inf, so the
+    pixel stays finite (the guide's parameters play no part: the BSDF branch
+    and its direction come from the path's own random numbers)."""
+    import torch
+
+    import wayland_amd as wl
+    res = 1024
+    print("case: glossy caustic 1024^2, guided, frame (19 << 20) + 62", flush=True)
+    sc = scenes.glossy_caustic(res, res)
+    pt = wl.PathTracer(gpu_ctx, wl.Accel(gpu_ctx, sc), sc)
+    guide = wl.Guide(gpu_ctx, seed=20)
+    pt.set_guide(guide, wl.GUIDE_SAMPLE)
+    cfg = wl.pt_config(res, res, spp=1, max_depth=16, frame_offset=(19 << 20) + 62,
+                       max_batch_paths=res * res)
+    film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
+    pt.render(cfg, film, stats=False)
+    pt.set_guide(None)
+    guide.close()
+    img = pt.resolve(film, 1).cpu().numpy().reshape(res * res, 3)
+    print(f"pixel 895139: {img[895139]}")
+    assert np.isfinite(img[895139]).all()
+    assert np.isfinite(img).all()
+    pt.close()

@@ -650,14 +650,21 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                         // the diffuse term below the surface, where the
                         // sampler draws nothing (trans = 0, clearcoat = 0)
                         const float cl = dot(nd, Ns);
+                        const float pb = pdf;
                         float sdens = pdf;
                         if (cl < 0.0f)
                             sdens -= 0.5f * (1.0f - mat.metallic) * (-cl * (1.0f / kPi));
                         pdf = P.galpha * guide_pdf(gm, nd) + (1.0f - P.galpha) * pdf;
                         if (!(pdf > 0.0f))
                             pdf = -1.0f; // a zero mixed density ends the path
+                        else if (__builtin_isinf(pb))
+                            // the reference pdf is infinite where its GGX
+                            // reflection folds back onto -V (1 / |L.H|, and
+                            // its sampler does land there): the limit of
+                            // s / p_mix as p_bsdf grows, not inf / inf
+                            missw = 1.0f / (1.0f - P.galpha);
                         else
-                            missw = fmaxf(sdens, 0.0f) / pdf;
+                            missw = fmaxf(sdens, 0.0f) / pdf; // 0 if only q is infinite
                     }
                     if (pdf < 0.0f || __builtin_isnan(pdf))
                     {
