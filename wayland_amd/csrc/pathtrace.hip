@@ -4,12 +4,15 @@
 // PathTracing.cu:40-139, __closesthit__radiance :181-289, __miss__radiance
 // :163-171, NEE :81-128, RR :130-138) as queue kernels:
 //
-//   raygen  -> [extend -> shade -> shadow] x max_depth -> accumulate
+//   [extend -> shade -> shadow] x max_depth -> accumulate
 //
-//   raygen     TEA seed (pixel, frame), pinhole camera ray, zeroed radiance
-//              (the first shade pass recomputes the rest of the first
-//              vertex's state instead of reading it).
-//   extend     closest-hit BVH8 traversal of the active ray queue.
+//   extend     closest-hit BVH8 traversal of the active ray queue.  The
+//              first pass of a single-scene batch traces path i's camera
+//              ray (TEA seed (pixel, frame), pinhole ray) computed in place,
+//              and the first shade pass recomputes that vertex's state and
+//              starts the path's radiance: nothing is written per path
+//              before the first traversal.  Multi-request batches keep a
+//              raygen pass (k_raygen_multi).
 //   shade      miss / emitter hit with MIS / depth cap / Disney BSDF
 //              sample-pdf-eval / NEE light sample + BSDF eval / RR, in the
 //              reference's RNG draw order; appends the continuing paths and
@@ -88,7 +91,7 @@ __device__ __forceinline__ void block_append2(BlockAppendLds &L, uint32_t pass,
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 __device__ __forceinline__ V3 v3of(float4 a) { return mk(a.x, a.y, a.z); }
 
-// ---- raygen -------------------------------------------------------------
+// ---- camera rays ----------------------------------------------------------
 // Camera ray of path p of a batch: its pixel and sample, the seed after
 // PinholeGenerateRay's two draws (Camera.h:28-35: u, then v), the direction.
 __device__ __forceinline__ void camera_ray(const PtParams &P, uint32_t p, V3 &o, V3 &d,
@@ -104,30 +107,19 @@ __device__ __forceinline__ void camera_ray(const PtParams &P, uint32_t p, V3 &o,
     o = ld3(P.cam.pos);
 }
 
-// Single-scene batches: the camera rays and zeroed radiance.  The rest of
-// the first vertex's path state (origin, direction and seed, throughput 1,
-// depth 0, lastTraceTerm 1, lastPdf 1) is not stored: the first shade pass
-// recomputes it with camera_ray, saving 64 B of writes and reads per path.
-__global__ __launch_bounds__(kPtBlock) void k_raygen(PtParams P, PtBuffers B)
+// Start of a single-scene batch: the first queue's length and the material
+// bins.  No per-path state is written: the first extend pass computes each
+// path's camera ray with camera_ray, and the first shade pass recomputes the
+// first vertex's state (origin, direction and seed, throughput 1, depth 0,
+// lastTraceTerm 1, lastPdf 1) and writes the path's radiance (and, when
+// training a guide, its record accumulator), so a path costs no ray, queue
+// or radiance write and read before its first traversal.
+__global__ void k_batch_start(PtParams P, PtBuffers B)
 {
-    const uint32_t npaths = P.npix * P.nsamples;
-    for (uint32_t p = blockIdx.x * kPtBlock + threadIdx.x; p < npaths;
-         p += gridDim.x * kPtBlock)
-    {
-        V3 o, d;
-        uint32_t seed;
-        camera_ray(P, p, o, d, seed);
-        B.rad[p] = f4(mk(0, 0, 0), 0.0f);
-        B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
-        B.qpath0[p] = p;
-        if (P.gmode & 2u)
-            B.gacc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        B.counters[0] = npaths;
-    if (blockIdx.x == 0 && P.sort_bins)
-        for (uint32_t b = threadIdx.x; b < P.sort_bins; b += kPtBlock)
-            B.sort_bins[b] = 0;
+    if (threadIdx.x == 0)
+        B.counters[0] = P.npix * P.nsamples;
+    for (uint32_t b = threadIdx.x; b < P.sort_bins; b += blockDim.x)
+        B.sort_bins[b] = 0;
 }
 
 // ---- top BVH levels in LDS ---------------------------------------------------
@@ -185,6 +177,35 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
         trav_persistent<false, kPtBlock, false, false, false, 0, 20>(
             S.nodes, S.tris, count, stack + threadIdx.x, load, store,
             reinterpret_cast<const WlBvh8Node *>(top), ntop);
+}
+
+// The first bounce of a single-scene batch: queue entry i is path i, and its
+// camera ray is computed here (the same camera_ray as the shade pass uses),
+// not read from a ray queue: nothing is written before the first traversal.
+template <int STACK>
+__global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_extend_first(PtParams P, PtScene S, PtBuffers B)
+{
+    __shared__ uint32_t stack[STACK * kPtBlock];
+    __shared__ uint4 top[8 * (pt_top_nodes<STACK, false>() ? pt_top_nodes<STACK, false>() : 1)];
+    const uint32_t ntop = pt_load_top<STACK, false>(top, S);
+    const uint32_t count = P.npix * P.nsamples;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        B.counters[1] = 0; // next queue
+        B.counters[2] = 0; // shadow queue
+        B.stats[0] += count;
+    }
+    auto load = [&](uint32_t i, TravRay &r) {
+        V3 o, d;
+        uint32_t seed;
+        camera_ray(P, i, o, d, seed);
+        trav_setup(r, o.x, o.y, o.z, d.x, d.y, d.z, P.epsilon, 1e30f);
+    };
+    auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
+    trav_persistent<false, kPtBlock, false, false, false, 0, 20>(
+        S.nodes, S.tris, count, stack + threadIdx.x, load, store,
+        reinterpret_cast<const WlBvh8Node *>(top), ntop);
 }
 
 // ---- material sort of the shade queue ------------------------------------------
@@ -382,8 +403,22 @@ void k_guide_infer_pt(PtParams P, PtScene S, PtBuffers B, int parity)
         if (valid)
         {
             const float4 hit = reinterpret_cast<const float4 *>(B.hits)[i];
-            const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
+            float4 r0, r1;
+            if (SEL == 1)
+            {
+                // the first queue: path i's camera ray (no ray queue is written)
+                V3 co, cd;
+                uint32_t sd;
+                camera_ray(P, i, co, cd, sd);
+                r0 = f4(co, P.epsilon);
+                r1 = f4(cd, 1e30f);
+            }
+            else
+            {
+                const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+                r0 = rp[0];
+                r1 = rp[1];
+            }
             const uint32_t prim = __float_as_uint(hit.y);
             const float4 *sr = reinterpret_cast<const float4 *>(S.prims) + 3 * (size_t)prim;
             const float4 w0 = sr[0], w1 = sr[1], w2 = sr[2];
@@ -467,7 +502,9 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
         V3 contrib = mk(0, 0, 0);
         if (in_range)
         {
-            path = qin[i];
+            // a single-scene batch's first queue is the identity (no raygen)
+            const bool ident = !MULTI && first;
+            path = ident ? i : qin[i];
             if (MULTI)
                 req = (parity ? B.qreq1 : B.qreq0)[i];
             const PtReq *rq = MULTI ? reqs + req : nullptr;
@@ -536,7 +573,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 // expectation when a guided vertex sampled the ray: the
                 // reference sampler's density over the mixed density
                 // there, and exactly 1 after every other vertex.
-                radiance = v3of(B.rad[path]);
+                radiance = ident ? mk(0, 0, 0) : v3of(B.rad[path]);
                 rad_in = radiance;
                 radiance = radiance + throughput * ld3(P.bg) * s_ltt.w;
                 rad_dirty = true;
@@ -582,7 +619,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                             neePdf = pdf * dist * dist / fabsf(dot(Ns, rayDir));
                         }
                         const float uptPdf = lastPdf;
-                        radiance = v3of(B.rad[path]);
+                        radiance = ident ? mk(0, 0, 0) : v3of(B.rad[path]);
                         rad_in = radiance;
                         rad_dirty = true;
                         radiance = radiance + throughput * ld3(lt.L) * lastTraceTerm *
@@ -785,7 +822,7 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                     }
                 }
             }
-            if (rad_dirty)
+            if (rad_dirty || ident) // the first pass starts every path's radiance
                 B.rad[path] = f4(radiance, 0.0f);
             if (gtrain)
             {
@@ -800,6 +837,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
                 }
                 if (grec_now) // nothing gathered yet; w marks the record
                     B.gacc[path] = make_float4(0.f, 0.f, 0.f, 1.0f);
+                else if (ident) // no record yet
+                    B.gacc[path] = make_float4(0.f, 0.f, 0.f, 0.f);
                 shadow_gacc = grec_done;
             }
             if (!done)
@@ -1146,13 +1185,20 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
 {
     const uint32_t npaths = P.npix * P.nsamples;
     const uint32_t blocks_all = (npaths + kPtBlock - 1) / kPtBlock;
-    const uint32_t g_gen = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
     // a BVH at most 16 deep needs at most 15 stack entries per lane
     const bool shallow = S.bvh_depth <= 16;
     // persistent traversal grids: the workgroups that fit the GPU at once
-    static int occ16 = 0, occ32 = 0;
+    static int occ16 = 0, occ32 = 0, occf16 = 0, occf32 = 0;
     if (!occ16)
     {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occf16, k_extend_first<16>, kPtBlock,
+                                                         0) != hipSuccess ||
+            occf16 < 1)
+            occf16 = 4;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occf32, k_extend_first<WL_BVH8_STACK_DEPTH>, kPtBlock, 0) != hipSuccess ||
+            occf32 < 1)
+            occf32 = 4;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ16, k_extend<16, false>, kPtBlock,
                                                          0) != hipSuccess ||
             occ16 < 1)
@@ -1164,16 +1210,24 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
     }
     const uint32_t g_trav =
         std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (uint32_t)(shallow ? occ16 : occ32));
+    const uint32_t g_first =
+        std::min<uint32_t>(blocks_all, (uint32_t)num_cus * (uint32_t)(shallow ? occf16 : occf32));
     const uint32_t g_shade = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 8);
     const uint32_t g_sort = std::min<uint32_t>(blocks_all, (uint32_t)num_cus * 4);
     // guide inference: 64 queries per workgroup pass, up to 4 workgroups per CU
     const uint32_t g_guide =
         std::max<uint32_t>(1, std::min<uint32_t>((npaths + 63) / 64, (uint32_t)num_cus * 4));
-    hipLaunchKernelGGL(k_raygen, dim3(g_gen), dim3(kPtBlock), 0, st, P, B);
+    hipLaunchKernelGGL(k_batch_start, dim3(1), dim3(kPtBlock), 0, st, P, B);
     for (uint32_t b = 0; b < P.max_depth; ++b)
     {
         const int parity = (int)(b & 1u);
-        if (shallow)
+        if (b == 0 && shallow)
+            hipLaunchKernelGGL((k_extend_first<16>), dim3(g_first), dim3(kPtBlock), 0, st, P, S,
+                               B);
+        else if (b == 0)
+            hipLaunchKernelGGL((k_extend_first<WL_BVH8_STACK_DEPTH>), dim3(g_first),
+                               dim3(kPtBlock), 0, st, P, S, B);
+        else if (shallow)
             hipLaunchKernelGGL((k_extend<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
                                parity, nullptr);
         else
