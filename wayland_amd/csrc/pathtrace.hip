@@ -149,7 +149,7 @@ __device__ __forceinline__ uint32_t pt_load_top(uint4 *top, const PtScene &S)
 // ---- extend: closest hit for the active queue -----------------------------
 template <int STACK, bool MULTI>
 __global__ __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(STACK == 16 ? 6 : 5)))
-void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs)
+void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs, float eps)
 {
     __shared__ uint32_t stack[STACK * kPtBlock];
     __shared__ uint4 top[8 * (pt_top_nodes<STACK, MULTI>() ? pt_top_nodes<STACK, MULTI>() : 1)];
@@ -163,7 +163,19 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
     const uint32_t *qreq = parity ? B.qreq1 : B.qreq0; // multi-request batches
-    auto load = [&](uint32_t i, TravRay &r) { load_ray_nt(rays, i, r); };
+    // single-scene queues carry the path's lastPdf and seed in the rays' w
+    // words (the shade pass reads its state back from the ray): the interval
+    // is [eps, 1e30] for every extension ray of such a batch
+    auto load = [&](uint32_t i, TravRay &r) {
+        if constexpr (MULTI)
+            load_ray_nt(rays, i, r);
+        else
+        {
+            const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
+            const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
+            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, eps, 1e30f);
+        }
+    };
     auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
     if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
         trav_persistent<false, kPtBlock, false, true, false, 8, 20>(
@@ -537,8 +549,20 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             }
             else
             {
-                s_pos = B.pos[path];
-                s_dir = B.dir[path];
+                if (MULTI)
+                {
+                    s_pos = B.pos[path];
+                    s_dir = B.dir[path];
+                }
+                else
+                {
+                    // the ray this pass shades: origin and lastPdf, then the
+                    // direction and the seed (written so by the last pass)
+                    const float4 *rp =
+                        reinterpret_cast<const float4 *>((parity ? B.rays1 : B.rays0) + i);
+                    s_pos = rp[0];
+                    s_dir = rp[1];
+                }
                 s_thr = B.thr[path];
                 s_ltt = B.ltt[path];
             }
@@ -844,14 +868,21 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (!done)
             {
                 cont = true;
-                B.pos[path] = f4(rayPos, lastPdf);
-                B.dir[path] = f4(rayDir, __uint_as_float(seed));
+                if (MULTI)
+                {
+                    B.pos[path] = f4(rayPos, lastPdf);
+                    B.dir[path] = f4(rayDir, __uint_as_float(seed));
+                }
                 B.thr[path] = f4(throughput,
                                  __uint_as_float(depth | (gvis << 16) |
                                                  (grec_done || grec_now ? kGrecBit : 0u)));
                 B.ltt[path] = f4(lastTraceTerm, missw);
-                next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
-                                   rayDir.x, rayDir.y, rayDir.z, 1e30f };
+                if (MULTI)
+                    next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
+                                       rayDir.x, rayDir.y, rayDir.z, 1e30f };
+                else // the path's lastPdf and seed ride in the ray's w words
+                    next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, lastPdf,
+                                       rayDir.x, rayDir.y, rayDir.z, __uint_as_float(seed) };
             }
         }
         // ---- compaction (whole wave participates)
@@ -1229,10 +1260,10 @@ hipError_t pt_run_batch(const PtParams &P, const PtScene &S, const PtBuffers &B,
                                dim3(kPtBlock), 0, st, P, S, B);
         else if (shallow)
             hipLaunchKernelGGL((k_extend<16, false>), dim3(g_trav), dim3(kPtBlock), 0, st, S, B,
-                               parity, nullptr);
+                               parity, nullptr, P.epsilon);
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, false>), dim3(g_trav),
-                               dim3(kPtBlock), 0, st, S, B, parity, nullptr);
+                               dim3(kPtBlock), 0, st, S, B, parity, nullptr, P.epsilon);
         if (P.sort_bins)
         {
             hipLaunchKernelGGL(k_sort_count, dim3(g_sort), dim3(kPtBlock), 0, st, P, S, B, parity);
@@ -1322,10 +1353,10 @@ hipError_t pt_run_multi(const PtMultiLaunch &L, const PtBuffers &B, int num_cus,
         const int parity = (int)(b & 1u);
         if (L.stack16)
             hipLaunchKernelGGL((k_extend<16, true>), dim3(g_trav), dim3(kPtBlock), 0, st, SA, B,
-                               parity, L.reqs);
+                               parity, L.reqs, 0.0f);
         else
             hipLaunchKernelGGL((k_extend<WL_BVH8_STACK_DEPTH, true>), dim3(g_trav),
-                               dim3(kPtBlock), 0, st, SA, B, parity, L.reqs);
+                               dim3(kPtBlock), 0, st, SA, B, parity, L.reqs, 0.0f);
         hipLaunchKernelGGL((k_shade<false, false, true>), dim3(g_shade), dim3(kPtBlock), 0, st,
                            P0, S0, B, parity, L.reqs, b == 0 ? 1 : 0);
         if (L.stack16)
