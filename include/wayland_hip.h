@@ -292,7 +292,7 @@ wl_status wl_pt_launch_frame(wl_scene scene, const wl_pt_config *cfg,
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The batch
  * buffers (path state, ray / hit / shadow queues, queue counters) live in one
  * fixed device array of the library, sized for WL_PT_NOALLOC_MAX_BATCH_PATHS
- * paths per batch (about 61 MB, held on each device the library runs on
+ * paths per batch (about 55 MB, held on each device the library runs on
  * until the process ends); the call makes no hipMalloc / hipFree and no
  * memset node.  Batches are formed as in wl_pt_render (one lane, no stats):
  * cfg->max_batch_paths = 0 means WL_PT_NOALLOC_MAX_BATCH_PATHS, a larger value

@@ -80,10 +80,14 @@ def _relmse(img, ref, trim=0.0):
 def test_guided_glossy_caustic_gain(gpu_ctx):
     """Glossy caustic at 256 x 256: guided PT (training included) has a
     lower relMSE than plain PT at the same sample count against a converged
-    plain reference."""
+    plain reference.  Three guides trained from different seeds (training
+    order also varies with the order of the collected records): over 20 such
+    runs the guided / plain ratio lay in 0.23 to 0.41, but one training in
+    about 20 ends markedly worse (0.82 seen once), so the median of the three
+    is tested."""
     import torch
     import wayland_amd as wl
-    print("case: glossy caustic 256x256, guided vs plain", flush=True)
+    print("case: glossy caustic 256x256, guided (3 seeds) vs plain", flush=True)
     res, spp = 256, 64
     sc = scenes.glossy_caustic(res, res)
     accel = wl.Accel(gpu_ctx, sc)
@@ -92,7 +96,7 @@ def test_guided_glossy_caustic_gain(gpu_ctx):
     film, _ = pt.render(cfg_ref, stats=False)
     ref = pt.resolve(film, 8192).cpu().numpy()
 
-    def run(guided):
+    def run():
         cfg = wl.pt_config(res, res, spp=1, max_depth=16, max_batch_paths=res * res)
         film = torch.zeros((res * res, 6), dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
@@ -103,19 +107,23 @@ def test_guided_glossy_caustic_gain(gpu_ctx):
         torch.cuda.synchronize()
         return pt.resolve(film, spp).cpu().numpy(), time.perf_counter() - t0
 
-    img_p, t_p = run(False)
-    guide = wl.Guide(gpu_ctx)
-    pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
-    img_g, t_g = run(True)
-    pt.set_guide(None)
-    for trim in (0.0, 0.001):
-        e_p, e_g = _relmse(img_p, ref, trim), _relmse(img_g, ref, trim)
-        print(f"trim {trim}: relMSE plain {e_p:.4e} ({t_p:.3f} s), guided {e_g:.4e} "
-              f"({t_g:.3f} s), ratio {e_g / e_p:.3f}, "
-              f"equal-time ratio {e_g * t_g / (e_p * t_p):.3f}")
-    # the firefly-robust error: guided below plain at the same sample count
-    assert _relmse(img_g, ref, 0.001) < 0.7 * _relmse(img_p, ref, 0.001)
-    guide.close()
+    img_p, t_p = run()
+    e_p = _relmse(img_p, ref, 0.001)
+    ratios = []
+    for seed in (1, 2, 3):
+        guide = wl.Guide(gpu_ctx, seed=seed)
+        pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
+        img_g, t_g = run()
+        pt.set_guide(None)
+        guide.close()
+        assert np.isfinite(img_g).all()
+        e_g = _relmse(img_g, ref, 0.001)
+        ratios.append(e_g / e_p)
+        print(f"seed {seed}: relMSE (trim 0.001) plain {e_p:.4e} ({t_p:.3f} s), guided "
+              f"{e_g:.4e} ({t_g:.3f} s), ratio {e_g / e_p:.3f}, untrimmed ratio "
+              f"{_relmse(img_g, ref) / _relmse(img_p, ref):.3f}")
+    # the firefly-robust error: guided well below plain at the same sample count
+    assert float(np.median(ratios)) < 0.6, ratios
 
 
 def test_guided_vertex_with_infinite_bsdf_pdf_stays_finite(gpu_ctx):

@@ -163,26 +163,23 @@ void k_extend(PtScene S, PtBuffers B, int parity, const PtReq *__restrict__ reqs
     }
     const wl_ray *rays = parity ? B.rays1 : B.rays0;
     const uint32_t *qreq = parity ? B.qreq1 : B.qreq0; // multi-request batches
-    // single-scene queues carry the path's lastPdf and seed in the rays' w
-    // words (the shade pass reads its state back from the ray): the interval
-    // is [eps, 1e30] for every extension ray of such a batch
-    auto load = [&](uint32_t i, TravRay &r) {
-        if constexpr (MULTI)
-            load_ray_nt(rays, i, r);
-        else
-        {
-            const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
-            const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
-            trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, eps, 1e30f);
-        }
+    // the queues carry the path's lastPdf and seed in the rays' w words (the
+    // shade pass reads its state back from the ray): every extension ray is
+    // traced on [epsilon, 1e30], epsilon the batch's or the ray's request's
+    auto load_eps = [&](uint32_t i, TravRay &r, float e) {
+        const f4v *rp = reinterpret_cast<const f4v *>(rays + i);
+        const f4v a = __builtin_nontemporal_load(rp), b = __builtin_nontemporal_load(rp + 1);
+        trav_setup(r, a.x, a.y, a.z, b.x, b.y, b.z, e, 1e30f);
     };
+    auto load = [&](uint32_t i, TravRay &r) { load_eps(i, r, eps); };
     auto store = [&](uint32_t i, const TravResult &h) { store_hit_nt(B.hits, i, h); };
     if constexpr (MULTI) // S.nodes / S.tris: the arena of the requests' BVHs
         trav_persistent<false, kPtBlock, false, true, false, 8, 20>(
             S.nodes, S.tris, count, stack + threadIdx.x,
             [&](uint32_t i, TravRay &r, uint32_t &root) {
-                root = reqs[qreq[i]].root;
-                load(i, r);
+                const PtReq &rq = reqs[qreq[i]];
+                root = rq.root;
+                load_eps(i, r, rq.P.epsilon);
             },
             store);
     else
@@ -514,8 +511,8 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
         V3 contrib = mk(0, 0, 0);
         if (in_range)
         {
-            // a single-scene batch's first queue is the identity (no raygen)
-            const bool ident = !MULTI && first;
+            // the first queue is the identity: entry i is path i
+            const bool ident = first != 0;
             path = ident ? i : qin[i];
             if (MULTI)
                 req = (parity ? B.qreq1 : B.qreq0)[i];
@@ -528,13 +525,15 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             float4 s_pos, s_dir, s_thr, s_ltt;
             if (first)
             {
-                // the first vertex: the state raygen did not store (the
-                // multi-request raygen keeps the direction and seed, whose
-                // recomputation would need the item lookup)
+                // the first vertex: single-scene batches recompute the camera
+                // ray; multi-request ones read it (with lastPdf 1 and the
+                // seed in its w words) from the queue the raygen pass wrote,
+                // whose item lookup the shade pass would otherwise repeat
                 if (MULTI)
                 {
-                    s_pos = f4(ld3(P.cam.pos), 1.0f);
-                    s_dir = B.dir[path];
+                    const float4 *rp = reinterpret_cast<const float4 *>(B.rays0 + i);
+                    s_pos = rp[0];
+                    s_dir = rp[1];
                 }
                 else
                 {
@@ -549,20 +548,12 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             }
             else
             {
-                if (MULTI)
-                {
-                    s_pos = B.pos[path];
-                    s_dir = B.dir[path];
-                }
-                else
-                {
-                    // the ray this pass shades: origin and lastPdf, then the
-                    // direction and the seed (written so by the last pass)
-                    const float4 *rp =
-                        reinterpret_cast<const float4 *>((parity ? B.rays1 : B.rays0) + i);
-                    s_pos = rp[0];
-                    s_dir = rp[1];
-                }
+                // the ray this pass shades: origin and lastPdf, then the
+                // direction and the seed (written so by the last pass)
+                const float4 *rp =
+                    reinterpret_cast<const float4 *>((parity ? B.rays1 : B.rays0) + i);
+                s_pos = rp[0];
+                s_dir = rp[1];
                 s_thr = B.thr[path];
                 s_ltt = B.ltt[path];
             }
@@ -868,21 +859,13 @@ void k_shade(PtParams P0, PtScene S0, PtBuffers B, int parity, const PtReq *__re
             if (!done)
             {
                 cont = true;
-                if (MULTI)
-                {
-                    B.pos[path] = f4(rayPos, lastPdf);
-                    B.dir[path] = f4(rayDir, __uint_as_float(seed));
-                }
                 B.thr[path] = f4(throughput,
                                  __uint_as_float(depth | (gvis << 16) |
                                                  (grec_done || grec_now ? kGrecBit : 0u)));
                 B.ltt[path] = f4(lastTraceTerm, missw);
-                if (MULTI)
-                    next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, P.epsilon,
-                                       rayDir.x, rayDir.y, rayDir.z, 1e30f };
-                else // the path's lastPdf and seed ride in the ray's w words
-                    next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, lastPdf,
-                                       rayDir.x, rayDir.y, rayDir.z, __uint_as_float(seed) };
+                // the path's lastPdf and seed ride in the ray's w words
+                next_ray = wl_ray{ rayPos.x, rayPos.y, rayPos.z, lastPdf,
+                                   rayDir.x, rayDir.y, rayDir.z, __uint_as_float(seed) };
             }
         }
         // ---- compaction (whole wave participates)
@@ -996,12 +979,10 @@ __global__ __launch_bounds__(kPtBlock) void k_raygen_multi(PtMultiLaunch L, PtBu
         const float v = 2.f * ((float)(int)y + rnd(seed)) / (float)(int)P.height - 1.f;
         const V3 d = normalize(ld3(P.cam.lookat) + ld3(P.cam.right) * u + ld3(P.cam.up) * v);
         const V3 o = ld3(P.cam.pos);
-        // origin, throughput, depth and lastTraceTerm are recomputed by the
-        // first shade pass
-        B.dir[p] = f4(d, __uint_as_float(seed));
-        B.rad[p] = f4(mk(0, 0, 0), 0.0f);
-        B.rays0[p] = wl_ray{ o.x, o.y, o.z, P.epsilon, d.x, d.y, d.z, 1e30f };
-        B.qpath0[p] = p;
+        // the ray carries lastPdf 1 and the seed in its w words, as the
+        // shade pass writes its rays; the first shade pass takes path = p
+        // and starts the path's radiance and state
+        B.rays0[p] = wl_ray{ o.x, o.y, o.z, 1.0f, d.x, d.y, d.z, __uint_as_float(seed) };
         B.qreq0[p] = it.req;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)

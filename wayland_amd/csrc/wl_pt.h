@@ -83,8 +83,6 @@ struct PtParams
 // Path state (structure of float4 arrays, one entry per path) and queues.
 struct PtBuffers
 {
-    float4 *pos;      // ray origin, w = lastPdf
-    float4 *dir;      // ray direction, w = seed bits
     float4 *thr;      // throughput, w = depth (bits 0-15), guided vertices (16-31)
     float4 *ltt;      // lastTraceTerm, w = weight of a miss of the next ray
     float4 *rad;      // radiance

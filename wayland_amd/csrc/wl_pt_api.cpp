@@ -29,8 +29,6 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
         p += (bytes + 255) & ~(size_t)255;
         return (void *)q;
     };
-    B.pos = (float4 *)take(16 * cap);
-    B.dir = (float4 *)take(16 * cap);
     B.thr = (float4 *)take(16 * cap);
     B.ltt = (float4 *)take(16 * cap);
     B.rad = (float4 *)take(16 * cap);
@@ -56,7 +54,7 @@ PtBuffers pt_carve(void *base, size_t cap, uint32_t *counters, unsigned long lon
 size_t pt_batch_bytes(size_t cap)
 {
     auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return r(16 * cap) * 5 + r(32 * cap) * 3 + r(4 * cap) * 8 + r(16 * cap) * 2 + r(4 * 2048);
+    return r(16 * cap) * 3 + r(32 * cap) * 3 + r(4 * cap) * 8 + r(16 * cap) * 2 + r(4 * 2048);
 }
 
 PtScene pt_scene_view(wl_scene s)

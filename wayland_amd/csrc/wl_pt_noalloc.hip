@@ -25,7 +25,7 @@ constexpr size_t kHeaderBytes = 256;
 constexpr size_t r256(size_t b) { return (b + 255) & ~(size_t)255; }
 // the carve of wl::pt_carve for kMaxPaths paths (checked against
 // wl::pt_batch_bytes at set-up)
-constexpr size_t kBatchBytes = r256(16ull * kMaxPaths) * 5 + r256(32ull * kMaxPaths) * 3 +
+constexpr size_t kBatchBytes = r256(16ull * kMaxPaths) * 3 + r256(32ull * kMaxPaths) * 3 +
                                r256(4ull * kMaxPaths) * 8 + r256(16ull * kMaxPaths) * 2 +
                                r256(4ull * 2048);
 constexpr size_t kScratchBytes = kHeaderBytes + kBatchBytes;
