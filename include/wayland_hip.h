@@ -322,7 +322,12 @@ wl_status wl_film_resolve(wl_context ctx, const float *d_film, uint32_t npix,
  * among the path's first guided_vertices such vertices (0: all) are guided.  In
  * training mode every path records its first guided vertex, and after each
  * batch the recorded samples train the network for steps_per_batch Adam
- * steps on the KL objective -F/p log q (Mueller et al., TOG 2019).
+ * steps on the KL objective -F/p log q (Mueller et al., TOG 2019).  That
+ * training runs on the guide's own stream while the next batch renders: the
+ * render samples from a published copy of the network, updated at the end of
+ * each training batch, so batch k samples from the network trained through
+ * batch k - 2 (none of its own paths).  Every other guide call, and a render
+ * that does not train, first waits for a pending training and publishes it.
  *
  * Parameter layout (WL_GUIDE_PARAMS floats): for layer l = 0..4 the weights
  * W_l[out][in] row-major then the biases b_l[out]; out = 64 (l < 4) or 32.
@@ -366,7 +371,7 @@ wl_status wl_guide_destroy(wl_guide guide);
 wl_status wl_guide_set_params(wl_guide guide, const float *params, uint32_t n);
 wl_status wl_guide_get_params(wl_guide guide, float *params, uint32_t n);
 wl_status wl_guide_set_mix(wl_guide guide, float mix_alpha);
-/* Synchronises the guide's last stream. */
+/* Synchronises the device. */
 wl_status wl_guide_get_info(wl_guide guide, wl_guide_info *info);
 /* Network outputs of n queries: d_out = 32 floats per query (row order). */
 wl_status wl_guide_infer(wl_guide guide, const float *d_queries, uint32_t n, float *d_out,

@@ -129,7 +129,8 @@ wl_status wl_guide_create(wl_context ctx, const wl_guide_config *cfg, wl_guide *
         2 * (size_t)5 * 64 * cap, 2 * (size_t)(4 * 64 + 32) * cap, // staging
         4 * (size_t)cap,                                     // loss
         64,                                                  // counts
-        64 * (size_t)c.max_samples                           // training samples
+        64 * (size_t)c.max_samples,                          // training samples
+        16 * (size_t)wl::kGuideFwdVecs, 4 * P                // published network
     };
     size_t total = 0;
     for (size_t b : sizes)
@@ -141,8 +142,8 @@ wl_status wl_guide_create(wl_context ctx, const wl_guide_config *cfg, wl_guide *
         return wl::hip_error(e, "guide buffers");
     }
     char *p = (char *)g->d_alloc;
-    void *ptr[11];
-    for (int k = 0; k < 11; ++k)
+    void *ptr[13];
+    for (int k = 0; k < 13; ++k)
     {
         ptr[k] = p;
         p += align256(sizes[k]);
@@ -161,7 +162,20 @@ wl_status wl_guide_create(wl_context ctx, const wl_guide_config *cfg, wl_guide *
     D.train = (float *)ptr[10];
     D.cap = cap;
     D.train_cap = c.max_samples;
+    g->pub_fwd = ptr[11];
+    g->pub_params = (float *)ptr[12];
     wl::ctx_retain(ctx);
+    // default priority, like every stream of the library
+    e = hipStreamCreateWithFlags(&g->tstream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&g->ev_pub, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&g->ev_trained, hipEventDisableTiming);
+    if (e != hipSuccess)
+    {
+        wl_guide_destroy(g);
+        return wl::hip_error(e, "guide training stream");
+    }
     std::vector<float> h(P);
     init_params(c.seed, h.data());
     st = wl_guide_set_params(g, h.data(), (uint32_t)P);
@@ -186,6 +200,12 @@ wl_status wl_guide_destroy(wl_guide g)
         return set_error(WL_ERROR_INVALID_VALUE, "guide is NULL");
     hipSetDevice(g->ctx->device);
     hipDeviceSynchronize();
+    if (g->ev_trained)
+        hipEventDestroy(g->ev_trained);
+    if (g->ev_pub)
+        hipEventDestroy(g->ev_pub);
+    if (g->tstream)
+        hipStreamDestroy(g->tstream);
     hipFree(g->d_alloc);
     wl_context ctx = g->ctx;
     delete g;
@@ -206,6 +226,11 @@ wl_status wl_guide_set_params(wl_guide g, const float *params, uint32_t n)
     WL_HIP_CHECK(hipMemset(g->dev.adam_v, 0, 4 * (size_t)n));
     WL_HIP_CHECK(wl::guide_pack(g->dev, nullptr));
     WL_HIP_CHECK(hipDeviceSynchronize());
+    g->pending = false; // the device is idle: any training has ended
+    wl_status st = wl::guide_publish(g, nullptr);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(hipDeviceSynchronize());
     g->steps = 0;
     return WL_SUCCESS;
 }
@@ -217,6 +242,9 @@ wl_status wl_guide_get_params(wl_guide g, float *params, uint32_t n)
     if (n != (uint32_t)wl::kGuideParams)
         return set_error(WL_ERROR_INVALID_VALUE, "n must be WL_GUIDE_PARAMS");
     WL_HIP_CHECK(hipSetDevice(g->ctx->device));
+    wl_status st = wl::guide_settle(g, nullptr);
+    if (st != WL_SUCCESS)
+        return st;
     WL_HIP_CHECK(hipDeviceSynchronize());
     WL_HIP_CHECK(hipMemcpy(params, g->dev.params, 4 * (size_t)n, hipMemcpyDeviceToHost));
     return WL_SUCCESS;
@@ -237,6 +265,9 @@ wl_status wl_guide_get_info(wl_guide g, wl_guide_info *info)
     if (!g || !info)
         return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
     WL_HIP_CHECK(hipSetDevice(g->ctx->device));
+    wl_status st = wl::guide_settle(g, nullptr);
+    if (st != WL_SUCCESS)
+        return st;
     WL_HIP_CHECK(hipDeviceSynchronize());
     uint32_t c[4];
     WL_HIP_CHECK(hipMemcpy(c, g->dev.count, sizeof c, hipMemcpyDeviceToHost));
@@ -252,6 +283,9 @@ wl_status wl_guide_infer(wl_guide g, const float *d_queries, uint32_t n, float *
     if (!g || (n && (!d_queries || !d_out)))
         return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
     WL_HIP_CHECK(hipSetDevice(g->ctx->device));
+    wl_status st = wl::guide_settle(g, (hipStream_t)stream);
+    if (st != WL_SUCCESS)
+        return st;
     WL_HIP_CHECK(wl::guide_infer(g->dev, d_queries, n, d_out, (hipStream_t)stream));
     return WL_SUCCESS;
 }
@@ -285,6 +319,9 @@ wl_status wl_guide_gradient(wl_guide g, const float *d_samples, uint32_t n, floa
         return set_error(WL_ERROR_LIMIT_EXCEEDED, "n exceeds the guide's batch_size");
     WL_HIP_CHECK(hipSetDevice(g->ctx->device));
     hipStream_t s = (hipStream_t)stream;
+    wl_status st = wl::guide_settle(g, s);
+    if (st != WL_SUCCESS)
+        return st;
     uint32_t *cnt = g->dev.count + 1;
     WL_HIP_CHECK(wl::guide_set_count(cnt, n, s));
     WL_HIP_CHECK(wl::guide_step(g->dev, d_samples, cnt, n, 0, std::max<uint32_t>(n, 1), nullptr,
@@ -317,6 +354,24 @@ wl_status guide_train_device(wl_guide g, const float *samples, const uint32_t *n
     g->last_stream = s;
     return WL_SUCCESS;
 }
+
+wl_status guide_publish(wl_guide g, hipStream_t s)
+{
+    WL_HIP_CHECK(hipMemcpyAsync(g->pub_fwd, g->dev.fwd, 16 * (size_t)kGuideFwdVecs,
+                                hipMemcpyDeviceToDevice, s));
+    WL_HIP_CHECK(hipMemcpyAsync(g->pub_params, g->dev.params, 4 * (size_t)kGuideParams,
+                                hipMemcpyDeviceToDevice, s));
+    return WL_SUCCESS;
+}
+
+wl_status guide_settle(wl_guide g, hipStream_t s)
+{
+    if (!g->pending)
+        return WL_SUCCESS;
+    WL_HIP_CHECK(hipStreamWaitEvent(s, g->ev_trained, 0));
+    g->pending = false;
+    return guide_publish(g, s);
+}
 } // namespace wl
 
 extern "C" wl_status wl_guide_train(wl_guide g, const float *d_samples, uint32_t n,
@@ -326,7 +381,13 @@ extern "C" wl_status wl_guide_train(wl_guide g, const float *d_samples, uint32_t
         return set_error(WL_ERROR_INVALID_VALUE, "NULL argument");
     WL_HIP_CHECK(hipSetDevice(g->ctx->device));
     hipStream_t s = (hipStream_t)stream;
+    wl_status st = wl::guide_settle(g, s);
+    if (st != WL_SUCCESS)
+        return st;
     uint32_t *cnt = g->dev.count + 1;
     WL_HIP_CHECK(wl::guide_set_count(cnt, n, s));
-    return wl::guide_train_device(g, d_samples, cnt, n, steps, s);
+    st = wl::guide_train_device(g, d_samples, cnt, n, steps, s);
+    if (st != WL_SUCCESS)
+        return st;
+    return wl::guide_publish(g, s);
 }

@@ -94,6 +94,16 @@ struct wl_guide_s
     void *d_alloc = nullptr;
     uint64_t steps = 0;
     hipStream_t last_stream = nullptr;
+    // The network the path tracer samples from: a copy of the packed forward
+    // operands and the FP32 parameters (biases), published from dev.fwd /
+    // dev.params.  A training render trains on `tstream` while its next
+    // batch renders from this copy; `pending` means a training is enqueued
+    // there (ending in ev_trained) whose result is not yet published.
+    void *pub_fwd = nullptr;
+    float *pub_params = nullptr;
+    hipStream_t tstream = nullptr;
+    hipEvent_t ev_pub = nullptr, ev_trained = nullptr;
+    bool pending = false;
 };
 
 namespace wl
@@ -127,6 +137,10 @@ wl_status noalloc_setup(int device);
 // Adam steps of a guide over samples[0, min(*n_dev, n_cap)) (wl_guide_api.cpp)
 wl_status guide_train_device(wl_guide g, const float *samples, const uint32_t *n_dev,
                              uint32_t n_cap, uint32_t steps, hipStream_t s);
+// copies the guide's current network to the path tracer's copy, on s
+wl_status guide_publish(wl_guide g, hipStream_t s);
+// orders s after the guide's pending training and publishes its result
+wl_status guide_settle(wl_guide g, hipStream_t s);
 } // namespace wl
 
 #define WL_HIP_CHECK(expr)                                                   \

@@ -199,19 +199,40 @@ wl_status ensure_lane2(wl_scene s, uint32_t paths)
     return WL_SUCCESS;
 }
 
-// Guided batches: after the bounces, move the training records into the
-// guide's buffer, train, and reset the sample count.
+// Guided training batches: after the bounces, wait for the training of the
+// previous batch (it reads the guide's sample buffer and writes the network)
+// and publish its network to the path tracer's copy, move this batch's
+// training records into the sample buffer, then train on them on the guide's
+// own stream, where the training overlaps the next batch's kernels.  Batch k
+// thus samples from the network trained through batch k - 2, which depends
+// on none of its own paths.
 wl_status after_batch(wl_scene s, const wl::PtParams &P, const wl::PtBuffers &B, hipStream_t hs)
 {
     if (!(P.gmode & WL_GUIDE_TRAIN))
         return WL_SUCCESS;
     wl_guide g = s->guide;
-    WL_HIP_CHECK(wl::pt_guide_collect(P, B, s->ctx->num_cus, hs));
-    wl_status st = wl::guide_train_device(g, g->dev.train, g->dev.count, g->dev.train_cap,
-                                          g->cfg.steps_per_batch, hs);
+    wl_status st = wl::guide_settle(g, hs);
     if (st != WL_SUCCESS)
         return st;
-    WL_HIP_CHECK(wl::guide_roll_count(g->dev.count, g->dev.train_cap, hs));
+    WL_HIP_CHECK(wl::pt_guide_collect(P, B, s->ctx->num_cus, hs));
+    WL_HIP_CHECK(hipEventRecord(g->ev_pub, hs));
+    WL_HIP_CHECK(hipStreamWaitEvent(g->tstream, g->ev_pub, 0));
+    st = wl::guide_train_device(g, g->dev.train, g->dev.count, g->dev.train_cap,
+                                g->cfg.steps_per_batch, g->tstream);
+    if (st != WL_SUCCESS)
+        return st;
+    WL_HIP_CHECK(wl::guide_roll_count(g->dev.count, g->dev.train_cap, g->tstream));
+    WL_HIP_CHECK(hipEventRecord(g->ev_trained, g->tstream));
+    g->pending = true;
+    return WL_SUCCESS;
+}
+
+// A guided render that does not train samples from the guide's latest
+// network: the render's stream waits for a pending training.
+wl_status before_render(wl_scene s, hipStream_t hs)
+{
+    if (s->guide && s->guide_mode && !(s->guide_mode & WL_GUIDE_TRAIN))
+        return wl::guide_settle(s->guide, hs);
     return WL_SUCCESS;
 }
 
@@ -230,8 +251,8 @@ wl::PtParams params_for(wl_scene s, const wl_pt_config *c)
         P.galpha = s->guide->cfg.mix_alpha;
         P.gbounces = s->guide->cfg.guided_bounces;
         P.gverts = s->guide->cfg.guided_vertices;
-        P.gfwd = s->guide->dev.fwd;
-        P.gparams = s->guide->dev.params;
+        P.gfwd = s->guide->pub_fwd;
+        P.gparams = s->guide->pub_params;
         std::memcpy(P.gbox_lo, s->box_lo, sizeof P.gbox_lo);
         std::memcpy(P.gbox_inv, s->box_inv, sizeof P.gbox_inv);
     }
@@ -629,6 +650,9 @@ wl_status wl_pt_render(wl_scene s, const wl_pt_config *c, float *film,
     if (st != WL_SUCCESS)
         return st;
     wl::PtBuffers B = carve(s);
+    st = before_render(s, hs);
+    if (st != WL_SUCCESS)
+        return st;
     wl::PtScene S = scene_view(s);
     if (stats)
         WL_HIP_CHECK(hipMemsetAsync(s->d_stats, 0, 16, hs));
@@ -693,6 +717,9 @@ wl_status wl_pt_render_samples(wl_scene s, const wl_pt_config *c, uint32_t frame
     if (st != WL_SUCCESS)
         return st;
     wl::PtBuffers B = carve(s);
+    st = before_render(s, hs);
+    if (st != WL_SUCCESS)
+        return st;
     for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
     {
         const uint32_t np = std::min(tile, npix_total - p0);
@@ -760,6 +787,9 @@ wl_status wl_pt_launch_frame(wl_scene s, const wl_pt_config *c, uint32_t frame,
     if (st != WL_SUCCESS)
         return st;
     wl::PtBuffers B = carve(s);
+    st = before_render(s, hs);
+    if (st != WL_SUCCESS)
+        return st;
     for (uint32_t p0 = 0; p0 < npix_total; p0 += tile)
     {
         const uint32_t np = std::min(tile, npix_total - p0);
@@ -786,6 +816,9 @@ wl_status wl_guide_infer_hits(wl_scene s, const wl_ray *rays, const wl_hit *hits
         return WL_SUCCESS;
     WL_HIP_CHECK(hipSetDevice(s->ctx->device));
     hipStream_t hs = (hipStream_t)stream;
+    wl_status st = wl::guide_settle(s->guide, hs); // the guide's latest network
+    if (st != WL_SUCCESS)
+        return st;
     wl_pt_config c{};
     c.width = 1;
     c.height = 1;
