@@ -154,3 +154,57 @@ def test_guided_vertex_with_infinite_bsdf_pdf_stays_finite(gpu_ctx):
     assert np.isfinite(img[895139]).all()
     assert np.isfinite(img).all()
     pt.close()
+
+
+def test_trained_network_is_published(gpu_ctx):
+    """A training render trains on the guide's own stream while its next
+    batch renders, and the tracer samples from a published copy of the
+    network.  After a training render, a render that does not train samples
+    from exactly the trained network: its film equals, bit for bit, the film
+    of a fresh guide given the same parameters, both when the sampling render
+    follows the training render directly and when get_params comes first;
+    inference sees the trained network too."""
+    import torch
+    import wayland_amd as wl
+    print("case: cornell 64x64, training render then sampling render", flush=True)
+    res = 64
+    sc = scenes.cornell_box(res, res)
+    pt = wl.PathTracer(gpu_ctx, wl.Accel(gpu_ctx, sc), sc)
+    cfg = wl.pt_config(res, res, spp=4, frame_offset=77, max_batch_paths=res * res)
+
+    def sample_film(guide):
+        pt.set_guide(guide, wl.GUIDE_SAMPLE)
+        film, _ = pt.render(cfg, stats=False)
+        return film.cpu().numpy().view(np.uint32)
+
+    def train(guide, frame0):
+        pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
+        tcfg = wl.pt_config(res, res, spp=4, frame_offset=frame0, max_batch_paths=res * res)
+        pt.render(tcfg, stats=False)  # 4 batches; the last one's training is pending
+
+    g = wl.Guide(gpu_ctx, seed=5, batch_size=4096, steps_per_batch=2)
+    fresh = wl.Guide(gpu_ctx, seed=6, batch_size=4096, steps_per_batch=2)
+    p0 = g.get_params()
+    train(g, 1 << 20)
+    f_direct = sample_film(g)  # straight after the training render
+    p1 = g.get_params()
+    assert g.info()["steps"] == 8
+    assert not np.array_equal(p0, p1)
+    fresh.set_params(p1)
+    assert np.array_equal(f_direct, sample_film(fresh))
+    train(g, 2 << 20)
+    p2 = g.get_params()  # waits for the pending training
+    f_after = sample_film(g)
+    fresh.set_params(p2)
+    assert np.array_equal(f_after, sample_film(fresh))
+    assert not np.array_equal(f_direct, f_after)
+    # inference reads the trained network as well
+    train(g, 3 << 20)
+    q = torch.rand((4096, 9), device="cuda") * 2 - 1
+    out_g = g.infer(q).cpu().numpy()
+    fresh.set_params(g.get_params())
+    assert np.array_equal(out_g.view(np.uint32), fresh.infer(q).cpu().numpy().view(np.uint32))
+    pt.set_guide(None)
+    g.close()
+    fresh.close()
+    pt.close()
