@@ -52,6 +52,8 @@ def parse():
                    help="guided glossy-caustic run length in spp (0 = skip)")
     p.add_argument("--guide-res", type=int, default=1024)
     p.add_argument("--guide-runs", type=int, default=60)
+    p.add_argument("--guide-steps", type=int, default=0,
+                   help="Adam steps per pass (0: the library default)")
     p.add_argument("--serve-requests", type=int, default=64,
                    help="batched-serving requests (0 = skip)")
     p.add_argument("--scene-file-frames", type=int, default=64,
@@ -243,7 +245,8 @@ def bench_guide(args, ctx, wl, torch):
                 relmse(gi, grid[0], mask=sphere))
 
     # allocate the per-path guide buffers outside the timed runs
-    warm_guide = wl.Guide(ctx)
+    gkw = {"steps_per_batch": args.guide_steps} if args.guide_steps else {}
+    warm_guide = wl.Guide(ctx, **gkw)
     pt.set_guide(warm_guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
     progressive(1, 1 << 28)
     pt.set_guide(None)
@@ -252,7 +255,7 @@ def bench_guide(args, ctx, wl, torch):
     runs = args.guide_runs
     g_sc, g_t, steps = [], [], 0
     for r in range(runs):
-        guide = wl.Guide(ctx, seed=r + 1)
+        guide = wl.Guide(ctx, seed=r + 1, **gkw)
         pt.set_guide(guide, wl.GUIDE_SAMPLE | wl.GUIDE_TRAIN)
         img_g, t_g = progressive(args.guide_spp, r << 20)
         steps = guide.info()["steps"]
