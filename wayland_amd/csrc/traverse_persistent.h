@@ -73,14 +73,23 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // coherent queues together.
 // top / ntop (optional): nodes [0, ntop) copied to LDS by the caller (see
 // node_visit).
+// DYN = 1: chunks of DCHUNK rays are handed out at run time instead: the
+// workgroup's chunk sequence is lb, lb + G, lb + 2G, ... (G workgroups), and
+// each of its waves takes the next chunk of that sequence from an LDS counter
+// (`dyn`, zeroed by the caller before a barrier) when its current one is used
+// up, so a wave slowed by expensive rays takes fewer chunks.  DYN = 2: one
+// global counter per XCD (`dyn`, 9 words: eight counters and an exit count,
+// all zero at launch and left zero by the last wave to exit): XCD x owns the
+// x-th eighth of the rays and its waves take that eighth's chunks in turn.
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
-          uint32_t CHUNKS = 0, int REFILL = kRefillLanes, class Fetch, class Emit>
+          uint32_t CHUNKS = 0, int REFILL = kRefillLanes, int DYN = 0,
+          uint32_t DCHUNK = 256, class Fetch, class Emit>
 __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ nodes,
                                                 const WlTriRecord *__restrict__ tris,
                                                 uint32_t n, uint32_t *stack_lane,
                                                 Fetch fetch, Emit emit,
                                                 const WlBvh8Node *top = nullptr,
-                                                uint32_t ntop = 0)
+                                                uint32_t ntop = 0, uint32_t *dyn = nullptr)
 {
     static_assert(BLOCK % 64 == 0, "whole waves only");
     const uint32_t waves = gridDim.x * (BLOCK / 64);
@@ -95,6 +104,44 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                                               : 64u;
     uint64_t cidx = wave;
     uint32_t next, end;
+    // DYN = 2: this wave's XCD, its eighth [xlo, xhi) of the rays and its
+    // index among the XCD's waves
+    const uint32_t xcd = blockIdx.x % 8u;
+    const uint32_t n8 = (n + 7u) / 8u;
+    const uint32_t xlo = xcd * n8 < n ? xcd * n8 : n;
+    const uint32_t xhi = xlo + n8 < n ? xlo + n8 : n;
+    const uint32_t wx = (blockIdx.x / 8u) * (BLOCK / 64) + threadIdx.x / 64;
+    const uint32_t wavesx = (G / 8u) * (BLOCK / 64);
+    // the next chunk of a DYN schedule into [next, end)
+    auto take_chunk = [&](bool first) {
+        uint64_t s0, lim = n;
+        if constexpr (DYN == 1)
+        {
+            uint32_t k = 0;
+            if (__lane_id() == 0)
+                k = atomicAdd(dyn, 1u);
+            k = __shfl(k, 0);
+            s0 = ((uint64_t)lb + (uint64_t)k * G) * DCHUNK;
+        }
+        else
+        {
+            uint32_t c = wx;
+            if (!first)
+            {
+                uint32_t k = 0;
+                if (__lane_id() == 0)
+                    k = atomicAdd(dyn + xcd, 1u);
+                c = wavesx + __shfl(k, 0);
+            }
+            s0 = xlo + (uint64_t)c * DCHUNK;
+            lim = xhi;
+        }
+        next = (uint32_t)(s0 < lim ? s0 : lim);
+        end = (uint32_t)(s0 + DCHUNK < lim ? s0 + DCHUNK : lim);
+    };
+    if constexpr (DYN != 0)
+        take_chunk(true);
+    else
     {
         const uint64_t s0 = cidx * C;
         next = (uint32_t)(s0 < n ? s0 : n);
@@ -129,7 +176,9 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                 }
             }
             next = next + nidle < end ? next + nidle : end;
-            if (CHUNKS != 0 && next >= end)
+            if (DYN != 0 && next >= end)
+                take_chunk(false);
+            else if (DYN == 0 && CHUNKS != 0 && next >= end)
             {
                 cidx += waves;
                 const uint64_t s0 = cidx * C;
@@ -154,6 +203,14 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
             emit(rid, st.res);
             alive = false;
         }
+    }
+    if constexpr (DYN == 2)
+    {
+        // the last wave to exit leaves the counters zero for the next launch
+        // (every wave has taken its last chunk before it counts itself out)
+        if (__lane_id() == 0 && atomicAdd(dyn + 8, 1u) == waves - 1u)
+            for (int k = 0; k < 9; ++k)
+                atomicExch(dyn + k, 0u);
     }
 }
 

@@ -39,7 +39,39 @@ void ctx_release(wl_context ctx)
         hipStreamSynchronize(ctx->stream);
         hipStreamDestroy(ctx->stream);
     }
+    if (!ctx->sched.empty())
+    {
+        hipDeviceSynchronize();
+        for (auto &kv : ctx->sched)
+            hipFree(kv.second);
+    }
     delete ctx;
+}
+
+hipError_t sched_counters(wl_context ctx, hipStream_t s, uint32_t **out)
+{
+    std::lock_guard<std::mutex> lock(ctx->sched_mu);
+    auto it = ctx->sched.find(s);
+    if (it != ctx->sched.end())
+    {
+        *out = it->second;
+        return hipSuccess;
+    }
+    uint32_t *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, 256);
+    if (e == hipSuccess)
+        e = hipMemset(p, 0, 256);
+    if (e == hipSuccess) // the zeros land before any stream's next launch
+        e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+    {
+        if (p)
+            hipFree(p);
+        return e;
+    }
+    ctx->sched.emplace(s, p);
+    *out = p;
+    return hipSuccess;
 }
 } // namespace wl
 
@@ -294,8 +326,6 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
     if (e == hipSuccess)
         e = hipMalloc(&a->d_tris, tbytes);
     if (e == hipSuccess)
-        e = hipMalloc(&a->d_counter, 256);
-    if (e == hipSuccess)
         e = hipMemcpyAsync(a->d_nodes, host.nodes.data(), nbytes,
                            hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && !host.tris.empty())
@@ -310,8 +340,6 @@ wl_status wl_accel_build_triangles(wl_context ctx, const wl_tri_input *inputs,
             hipFree(a->d_nodes);
         if (a->d_tris)
             hipFree(a->d_tris);
-        if (a->d_counter)
-            hipFree(a->d_counter);
         delete a;
         return wl::hip_error(e, "accel upload");
     }
@@ -344,7 +372,6 @@ wl_status wl_accel_destroy(wl_accel a)
     hipDeviceSynchronize();
     hipFree(a->d_nodes);
     hipFree(a->d_tris);
-    hipFree(a->d_counter);
     wl_context ctx = a->ctx;
     delete a;
     wl::ctx_release(ctx);

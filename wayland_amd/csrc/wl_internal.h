@@ -4,7 +4,9 @@
 
 #include <stdint.h>
 #include <atomic>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/wayland_hip.h"
@@ -20,12 +22,20 @@ struct wl_context_s
     std::string cache_path;
     int num_cus = 0;
     std::atomic<int> refs{ 1 };
+    // Chunk counters of the raw persistent trace kernels: one zeroed 256-B
+    // block per stream that launches them.  A launch leaves its block zero
+    // again, so launches on one stream reuse it in stream order and launches
+    // on different streams never share one.
+    std::mutex sched_mu;
+    std::unordered_map<hipStream_t, uint32_t *> sched;
 };
 
 namespace wl
 {
 void ctx_retain(wl_context ctx);
 void ctx_release(wl_context ctx);
+// The chunk counter block of stream s (allocated and zeroed on first use).
+hipError_t sched_counters(wl_context ctx, hipStream_t s, uint32_t **out);
 } // namespace wl
 
 struct wl_accel_s
@@ -33,8 +43,6 @@ struct wl_accel_s
     wl_context ctx = nullptr;
     WlBvh8Node *d_nodes = nullptr;
     WlTriRecord *d_tris = nullptr;
-    uint32_t *d_counter = nullptr; // ray counter of the persistent trace kernels
-    int occ_closest = 0, occ_any = 0; // resident workgroups per CU (queried once)
     uint32_t num_nodes = 0;
     uint64_t num_tris = 0;
     uint32_t num_tri_records = 0; // entries of d_tris (leaf triangle records)
