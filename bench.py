@@ -36,8 +36,11 @@ PT_RAY_HIT_BYTES, PT_SHADE_BYTES, PT_SHADOW_BYTES = 48, 380, 52
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    # 100 launches of about 1 ms after 20 untimed ones: the clocks settle
+    # within the warm-up (3 warm-up launches and 20 timed ones read 4-5 %
+    # lower)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--scene", default="25x6", help="COUNTxSUBDIV icospheres")
     p.add_argument("--rays", type=int, default=1 << 23)
     p.add_argument("--scene-b", default="64x7",

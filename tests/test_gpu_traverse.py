@@ -144,3 +144,46 @@ def test_64x7_scene_subsample_bit_exact(gpu_ctx):
     del rays, obvh
     accel.close()
     torch.cuda.empty_cache()
+
+
+def test_concurrent_streams_and_sizes(gpu_ctx):
+    """The raw trace kernels hand out chunks of rays at run time from counters
+    kept per stream: traces of one accel launched on two streams at once, and
+    launches of many sizes one after another on one stream (each advancing
+    the counters by its own chunk count), give the hits of a lone trace,
+    which are checked against the oracle on a subsample."""
+    import torch
+    import wayland_amd as wl
+    print("case: 8 icospheres subdiv 4, two streams, sizes 1 .. 2^20", flush=True)
+    sc = scenes.icosphere_scene(8, 4)
+    accel = wl.Accel(gpu_ctx, sc)
+    ra_np = scenes.traversal_rays(1 << 20, seed=5)
+    rb_np = scenes.traversal_rays((3 << 18) + 77, seed=6)
+    ha, oa = _gpu_hits(accel, ra_np)
+    hb, ob_ = _gpu_hits(accel, rb_np)
+    obvh = ob.OracleBVH(sc.triangles())
+    _assert_bit_exact(ha[::97], obvh.closest(ra_np[::97]))
+    ra = torch.from_numpy(ra_np).cuda()
+    rb = torch.from_numpy(rb_np).cuda()
+    main = torch.cuda.current_stream()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    s1.wait_stream(main)
+    s2.wait_stream(main)
+    outs = []
+    for _ in range(4):
+        with torch.cuda.stream(s1):
+            outs.append(("a", accel.trace_closest(ra), accel.trace_any(ra)))
+        with torch.cuda.stream(s2):
+            outs.append(("b", accel.trace_closest(rb), accel.trace_any(rb)))
+    torch.cuda.synchronize()
+    for which, h, bits in outs:
+        ref_h, ref_o, n = (ha, oa, ra.shape[0]) if which == "a" else (hb, ob_, rb.shape[0])
+        _assert_bit_exact(wl.hits_to_numpy(h), ref_h)
+        assert np.array_equal(wl.unpack_bits(bits, n), ref_o)
+    # sizes below, at and across the chunk and XCD boundaries, back to back
+    for n in (1, 7, 8, 255, 256, 257, 2047, 2048, 2049, 65537, 1 << 20):
+        h = wl.hits_to_numpy(accel.trace_closest(ra[:n].contiguous()))
+        o = wl.unpack_bits(accel.trace_any(ra[:n].contiguous()), n)
+        _assert_bit_exact(h, ha[:n])
+        assert np.array_equal(o, oa[:n])
+    accel.close()

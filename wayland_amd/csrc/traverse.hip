@@ -29,6 +29,12 @@ constexpr uint32_t kTopNodes = 73;
 // waves: 8.68-8.75 -> 9.32-9.41 Grays/s on 2^23 incoherent closest-hit rays.
 constexpr uint32_t kTraceChunk = 256;
 
+// The chunk counters' values at launch, one per XCD.
+struct TraceSched
+{
+    uint32_t base[8];
+};
+
 template <int STACK>
 constexpr uint32_t top_nodes() { return STACK == 16 ? kTopNodes : 0u; }
 
@@ -47,15 +53,15 @@ __device__ __forceinline__ void load_ray(const wl_ray *__restrict__ rays, uint32
     load_ray_nt(rays, i, r);
 }
 
-// ntop: nodes copied to LDS, min(node count, top_nodes<STACK>()); sched: the
-// launching stream's chunk counters (zero, and left zero).  The grid is a
-// multiple of 8.
+// ntop: nodes copied to LDS, min(node count, top_nodes<STACK>()); sched /
+// base: the launching stream's chunk counters and their values at launch.
+// The grid is a multiple of 8.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
 void k_trace_closest_p(
     const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
     const wl_ray *__restrict__ rays, uint32_t n, wl_hit *__restrict__ hits, uint32_t ntop,
-    uint32_t *__restrict__ sched)
+    uint32_t *__restrict__ sched, TraceSched base)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
     __shared__ uint4 top[8 * (top_nodes<STACK>() ? top_nodes<STACK>() : 1)];
@@ -65,30 +71,29 @@ void k_trace_closest_p(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) { store_hit_nt(hits, i, h); },
-        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u, sched);
+        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u, sched,
+        base.base[blockIdx.x % 8u]);
 }
-
-constexpr int kAnyDyn = 2;
 
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 void k_trace_any_p(const WlBvh8Node *__restrict__ nodes, const WlTriRecord *__restrict__ tris,
                    const wl_ray *__restrict__ rays, uint32_t n, uint32_t *__restrict__ bits,
-                   uint32_t ntop, uint32_t *__restrict__ sched)
+                   uint32_t ntop, uint32_t *__restrict__ sched, TraceSched base)
 {
     __shared__ uint32_t stack[STACK * kTraceBlock];
     __shared__ uint4 top[8 * (top_nodes<STACK>() ? top_nodes<STACK>() : 1)];
     if (top_nodes<STACK>())
         load_top(top, nodes, ntop);
-    trav_persistent<true, kTraceBlock, false, false, true, kAnyDyn ? 0 : 4, kRefillLanes,
-                    kAnyDyn, kTraceChunk>(
+    trav_persistent<true, kTraceBlock, false, false, true, 0, kRefillLanes, 2, kTraceChunk>(
         nodes, tris, n, stack + threadIdx.x,
         [&](uint32_t i, TravRay &r) { load_ray(rays, i, r); },
         [&](uint32_t i, const TravResult &h) {
             if (h.prim != WL_INVALID)
                 atomicOr(&bits[i >> 5], 1u << (i & 31u));
         },
-        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u, sched);
+        reinterpret_cast<const WlBvh8Node *>(top), top_nodes<STACK>() ? ntop : 0u, sched,
+        base.base[blockIdx.x % 8u]);
 }
 
 // Instrumented closest hit: one ray per lane, per-ray node/triangle counts.
@@ -141,6 +146,18 @@ static uint32_t persistent_grid(K kernel, uint32_t /*unused*/, int num_cus)
     return (uint32_t)(per_cu * std::max(1, num_cus));
 }
 
+// The counters' values for a launch over n rays, advanced past it.
+static TraceSched take_bases(wl_context_s::SchedBlock *sb, uint32_t n)
+{
+    TraceSched t;
+    for (uint32_t x = 0; x < 8; ++x)
+    {
+        t.base[x] = sb->base[x];
+        sb->base[x] += dyn_xcd_chunks(n, x, kTraceChunk);
+    }
+    return t;
+}
+
 // The persistent grid for n rays, rounded up to a multiple of 8 so that each
 // XCD's eighth of the rays has workgroups (chunk scheduling DYN = 2).
 static uint32_t xcd_grid(uint32_t grid_cache, uint32_t n)
@@ -161,14 +178,16 @@ static hipError_t closest_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t
         grid_cache = persistent_grid(k_trace_closest_p<STACK>, 0xFFFFFFFFu, a->ctx->num_cus);
         cus_cache = a->ctx->num_cus;
     }
-    uint32_t *sched = nullptr;
-    const hipError_t e = sched_counters(a->ctx, s, &sched);
+    const uint32_t grid = xcd_grid(grid_cache, n);
+    std::lock_guard<std::mutex> lock(a->ctx->sched_mu);
+    wl_context_s::SchedBlock *sb = nullptr;
+    const hipError_t e = sched_block(a->ctx, s, &sb);
     if (e != hipSuccess)
         return e;
-    const uint32_t grid = xcd_grid(grid_cache, n);
+    const TraceSched base = take_bases(sb, n);
     hipLaunchKernelGGL(k_trace_closest_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s,
                        a->d_nodes, a->d_tris, rays, n, hits,
-                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()), sched);
+                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()), sb->dev, base);
     return hipGetLastError();
 }
 
@@ -183,17 +202,19 @@ static hipError_t any_impl(const wl_accel_s *a, const wl_ray *rays, uint32_t n,
         grid_cache = persistent_grid(k_trace_any_p<STACK>, 0xFFFFFFFFu, a->ctx->num_cus);
         cus_cache = a->ctx->num_cus;
     }
-    uint32_t *sched = nullptr;
-    const hipError_t e = sched_counters(a->ctx, s, &sched);
+    const uint32_t grid = xcd_grid(grid_cache, n);
+    std::lock_guard<std::mutex> lock(a->ctx->sched_mu);
+    wl_context_s::SchedBlock *sb = nullptr;
+    const hipError_t e = sched_block(a->ctx, s, &sb);
     if (e != hipSuccess)
         return e;
-    const uint32_t grid = xcd_grid(grid_cache, n);
+    const TraceSched base = take_bases(sb, n);
     const uint32_t words = (n + 31) / 32;
     hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>((words + 255) / 256, 4096)),
                        dim3(256), 0, s, bits, words);
     hipLaunchKernelGGL(k_trace_any_p<STACK>, dim3(grid), dim3(kTraceBlock), 0, s, a->d_nodes,
                        a->d_tris, rays, n, bits,
-                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()), sched);
+                       std::min<uint32_t>(a->num_nodes, top_nodes<STACK>()), sb->dev, base);
     return hipGetLastError();
 }
 

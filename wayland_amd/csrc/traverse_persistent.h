@@ -78,9 +78,12 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // each of its waves takes the next chunk of that sequence from an LDS counter
 // (`dyn`, zeroed by the caller before a barrier) when its current one is used
 // up, so a wave slowed by expensive rays takes fewer chunks.  DYN = 2: one
-// global counter per XCD (`dyn`, 9 words: eight counters and an exit count,
-// all zero at launch and left zero by the last wave to exit): XCD x owns the
-// x-th eighth of the rays and its waves take that eighth's chunks in turn.
+// global counter per XCD (`dyn`, 8 words): XCD x owns the x-th eighth of the
+// rays, its waves start on chunks 0 .. W-1 of that eighth (W waves per XCD)
+// and then take chunk W + (counter - base) each, base = `dyn_base`, the
+// counter's value at launch.  Every wave that had a chunk makes exactly one
+// fetch past the eighth's last chunk, so a launch advances counter x by the
+// eighth's chunk count: the host keeps the bases and nothing is reset.
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
           uint32_t CHUNKS = 0, int REFILL = kRefillLanes, int DYN = 0,
           uint32_t DCHUNK = 256, class Fetch, class Emit>
@@ -89,7 +92,8 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                                                 uint32_t n, uint32_t *stack_lane,
                                                 Fetch fetch, Emit emit,
                                                 const WlBvh8Node *top = nullptr,
-                                                uint32_t ntop = 0, uint32_t *dyn = nullptr)
+                                                uint32_t ntop = 0, uint32_t *dyn = nullptr,
+                                                uint32_t dyn_base = 0)
 {
     static_assert(BLOCK % 64 == 0, "whole waves only");
     const uint32_t waves = gridDim.x * (BLOCK / 64);
@@ -130,7 +134,7 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
             {
                 uint32_t k = 0;
                 if (__lane_id() == 0)
-                    k = atomicAdd(dyn + xcd, 1u);
+                    k = atomicAdd(dyn + xcd, 1u) - dyn_base;
                 c = wavesx + __shfl(k, 0);
             }
             s0 = xlo + (uint64_t)c * DCHUNK;
@@ -204,14 +208,16 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
             alive = false;
         }
     }
-    if constexpr (DYN == 2)
-    {
-        // the last wave to exit leaves the counters zero for the next launch
-        // (every wave has taken its last chunk before it counts itself out)
-        if (__lane_id() == 0 && atomicAdd(dyn + 8, 1u) == waves - 1u)
-            for (int k = 0; k < 9; ++k)
-                atomicExch(dyn + k, 0u);
-    }
+}
+
+// Chunks of each XCD's eighth of n rays under DYN = 2 (the amount a launch
+// advances that XCD's counter by).
+__host__ __device__ inline uint32_t dyn_xcd_chunks(uint32_t n, uint32_t xcd, uint32_t chunk)
+{
+    const uint32_t n8 = (n + 7u) / 8u;
+    const uint32_t lo = xcd * n8 < n ? xcd * n8 : n;
+    const uint32_t hi = lo + n8 < n ? lo + n8 : n;
+    return (hi - lo + chunk - 1u) / chunk;
 }
 
 } // namespace wl

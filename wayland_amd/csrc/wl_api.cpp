@@ -43,34 +43,32 @@ void ctx_release(wl_context ctx)
     {
         hipDeviceSynchronize();
         for (auto &kv : ctx->sched)
-            hipFree(kv.second);
+            hipFree(kv.second.dev);
     }
     delete ctx;
 }
 
-hipError_t sched_counters(wl_context ctx, hipStream_t s, uint32_t **out)
+hipError_t sched_block(wl_context ctx, hipStream_t s, wl_context_s::SchedBlock **out)
 {
-    std::lock_guard<std::mutex> lock(ctx->sched_mu);
     auto it = ctx->sched.find(s);
     if (it != ctx->sched.end())
     {
-        *out = it->second;
+        *out = &it->second;
         return hipSuccess;
     }
-    uint32_t *p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, 256);
+    wl_context_s::SchedBlock b;
+    hipError_t e = hipMalloc((void **)&b.dev, 256);
     if (e == hipSuccess)
-        e = hipMemset(p, 0, 256);
+        e = hipMemset(b.dev, 0, 256);
     if (e == hipSuccess) // the zeros land before any stream's next launch
         e = hipDeviceSynchronize();
     if (e != hipSuccess)
     {
-        if (p)
-            hipFree(p);
+        if (b.dev)
+            hipFree(b.dev);
         return e;
     }
-    ctx->sched.emplace(s, p);
-    *out = p;
+    *out = &ctx->sched.emplace(s, b).first->second;
     return hipSuccess;
 }
 } // namespace wl

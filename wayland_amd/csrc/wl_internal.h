@@ -22,20 +22,27 @@ struct wl_context_s
     std::string cache_path;
     int num_cus = 0;
     std::atomic<int> refs{ 1 };
-    // Chunk counters of the raw persistent trace kernels: one zeroed 256-B
-    // block per stream that launches them.  A launch leaves its block zero
-    // again, so launches on one stream reuse it in stream order and launches
-    // on different streams never share one.
-    std::mutex sched_mu;
-    std::unordered_map<hipStream_t, uint32_t *> sched;
+    // Chunk counters of the raw persistent trace kernels: one block of eight
+    // device counters per stream that launches them, with the counters'
+    // values at the next launch on that stream (kept by the host: a launch
+    // advances them by known amounts).  Launches on one stream use the block
+    // in stream order; launches on different streams never share one.
+    struct SchedBlock
+    {
+        uint32_t *dev = nullptr;
+        uint32_t base[8] = {};
+    };
+    std::mutex sched_mu; // held from reading the bases to the launch
+    std::unordered_map<hipStream_t, SchedBlock> sched;
 };
 
 namespace wl
 {
 void ctx_retain(wl_context ctx);
 void ctx_release(wl_context ctx);
-// The chunk counter block of stream s (allocated and zeroed on first use).
-hipError_t sched_counters(wl_context ctx, hipStream_t s, uint32_t **out);
+// The chunk counter block of stream s (allocated on first use); the caller
+// holds ctx->sched_mu.
+hipError_t sched_block(wl_context ctx, hipStream_t s, wl_context_s::SchedBlock **out);
 } // namespace wl
 
 struct wl_accel_s
