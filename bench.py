@@ -682,7 +682,12 @@ def main():
                      "traffic_frac": (traffic / t_closest / 1e9 / HBM_PEAK_GBS
                                       if traffic else None),
                      "bytes_per_ray": bytes_per_ray, "nodes_per_ray": c[0],
-                     "tris_per_ray": c[1]},
+                     "tris_per_ray": c[1],
+                     "note": "achieved counts every node and triangle fetch (ray 32 B + "
+                             "hit 16 B + nodes x 128 B + triangles x 48 B per ray), also "
+                             "the top three BVH levels read from LDS and fetches that hit "
+                             "in L2 or the Infinity Cache, so frac may exceed 1; "
+                             "traffic_frac is the measured HBM-side share"},
         "cpu_baseline": cpu,
         "traversal_point_b": point_b,
         "pathtracer": pt_out,
