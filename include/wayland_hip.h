@@ -131,6 +131,12 @@ typedef struct wl_hit
     float b1, b2;
 } wl_hit;
 
+/* wl_trace_closest and wl_trace_any hand the rays out to the GPU's waves at
+ * run time from a small block of counters kept per stream (the context's):
+ * the first trace on a stream allocates that block (256 bytes, freed with the
+ * context) and synchronises the device, so a caller that captures traces
+ * into a graph makes one trace on that stream before capturing.  Traces on
+ * one stream are ordered; traces on different streams share nothing. */
 wl_status wl_trace_closest(wl_accel accel, const wl_ray *d_rays, uint32_t n,
                            wl_hit *d_hits, void *stream);
 /* Any hit with OPTIX_RAY_FLAG_TERMINATE_ON_FIRST_HIT semantics: bit (i % 32)
