@@ -73,17 +73,15 @@ __device__ __forceinline__ uint32_t lane_prefix(uint64_t m)
 // coherent queues together.
 // top / ntop (optional): nodes [0, ntop) copied to LDS by the caller (see
 // node_visit).
-// DYN = 1: chunks of DCHUNK rays are handed out at run time instead: the
-// workgroup's chunk sequence is lb, lb + G, lb + 2G, ... (G workgroups), and
-// each of its waves takes the next chunk of that sequence from an LDS counter
-// (`dyn`, zeroed by the caller before a barrier) when its current one is used
-// up, so a wave slowed by expensive rays takes fewer chunks.  DYN = 2: one
-// global counter per XCD (`dyn`, 8 words): XCD x owns the x-th eighth of the
-// rays, its waves start on chunks 0 .. W-1 of that eighth (W waves per XCD)
-// and then take chunk W + (counter - base) each, base = `dyn_base`, the
-// counter's value at launch.  Every wave that had a chunk makes exactly one
-// fetch past the eighth's last chunk, so a launch advances counter x by the
-// eighth's chunk count: the host keeps the bases and nothing is reset.
+// DYN = 2: chunks of DCHUNK rays are handed out at run time instead, so a
+// wave slowed by expensive rays takes fewer chunks.  One global counter per
+// XCD (`dyn`, 8 words): XCD x owns the x-th eighth of the rays, its waves
+// start on chunks 0 .. W-1 of that eighth (W waves per XCD; the grid is a
+// multiple of 8) and then take chunk W + (counter - base) each, base =
+// `dyn_base`, the counter's value at launch.  Every wave that had a chunk
+// makes exactly one fetch past the eighth's last chunk, so a launch advances
+// counter x by the eighth's chunk count: the host keeps the bases and nothing
+// is reset.
 template <bool ANY, int BLOCK, bool AUX, bool ROOTED = false, bool XCDMAP = false,
           uint32_t CHUNKS = 0, int REFILL = kRefillLanes, int DYN = 0,
           uint32_t DCHUNK = 256, class Fetch, class Emit>
@@ -108,40 +106,28 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
                                               : 64u;
     uint64_t cidx = wave;
     uint32_t next, end;
-    // DYN = 2: this wave's XCD, its eighth [xlo, xhi) of the rays and its
-    // index among the XCD's waves
+    // DYN: this wave's XCD, its eighth [xlo, xhi) of the rays and its index
+    // among the XCD's waves
     const uint32_t xcd = blockIdx.x % 8u;
     const uint32_t n8 = (n + 7u) / 8u;
     const uint32_t xlo = xcd * n8 < n ? xcd * n8 : n;
     const uint32_t xhi = xlo + n8 < n ? xlo + n8 : n;
     const uint32_t wx = (blockIdx.x / 8u) * (BLOCK / 64) + threadIdx.x / 64;
     const uint32_t wavesx = (G / 8u) * (BLOCK / 64);
-    // the next chunk of a DYN schedule into [next, end)
+    // the next chunk of the DYN schedule into [next, end)
     auto take_chunk = [&](bool first) {
-        uint64_t s0, lim = n;
-        if constexpr (DYN == 1)
+        static_assert(DYN == 0 || DYN == 2, "DYN is 0 or 2");
+        uint32_t c = wx;
+        if (!first)
         {
             uint32_t k = 0;
             if (__lane_id() == 0)
-                k = atomicAdd(dyn, 1u);
-            k = __shfl(k, 0);
-            s0 = ((uint64_t)lb + (uint64_t)k * G) * DCHUNK;
+                k = atomicAdd(dyn + xcd, 1u) - dyn_base;
+            c = wavesx + __shfl(k, 0);
         }
-        else
-        {
-            uint32_t c = wx;
-            if (!first)
-            {
-                uint32_t k = 0;
-                if (__lane_id() == 0)
-                    k = atomicAdd(dyn + xcd, 1u) - dyn_base;
-                c = wavesx + __shfl(k, 0);
-            }
-            s0 = xlo + (uint64_t)c * DCHUNK;
-            lim = xhi;
-        }
-        next = (uint32_t)(s0 < lim ? s0 : lim);
-        end = (uint32_t)(s0 + DCHUNK < lim ? s0 + DCHUNK : lim);
+        const uint64_t s0 = xlo + (uint64_t)c * DCHUNK;
+        next = (uint32_t)(s0 < xhi ? s0 : xhi);
+        end = (uint32_t)(s0 + DCHUNK < xhi ? s0 + DCHUNK : xhi);
     };
     if constexpr (DYN != 0)
         take_chunk(true);
