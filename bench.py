@@ -189,7 +189,10 @@ def bench_guide(args, ctx, wl, torch):
     wall time, both scored by relMSE against the committed 65,536-spp FP64
     oracle images: the whole view on a 256 x 256 pixel grid (every 4th pixel;
     the headline), its floor pixels, its pixels off and on the glass sphere,
-    and the 128 x 128 caustic window."""
+    and the 128 x 128 caustic window.  Plain PT runs twice per run index: with
+    its own frame offsets (independent of the guided run) and with the guided
+    run's (common random numbers: paths that meet no guided vertex are then
+    the same samples on both sides; `common_seeds`, compared run by run)."""
     from wayland_amd import scenes
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import make_fp64_references as fx
@@ -282,19 +285,37 @@ def bench_guide(args, ctx, wl, torch):
         if abs(ratio - 1.0) <= 0.03:
             break
         spp_p = max(1, int(round(spp_p * ratio)))
-    g_sc, p_sc = np.array(g_sc), np.array(p_sc)
+    # plain runs with the guided runs' frame offsets (common random numbers):
+    # paths that meet no guided vertex, such as the glass-only paths of the
+    # sphere's pixels, draw the same numbers on both sides, so their
+    # fireflies are shared by run i of each side instead of being independent
+    c_sc = []
+    for r in range(runs):
+        img_c, _ = progressive(spp_p, r << 20)
+        c_sc.append(scores(img_c))
+    g_sc, p_sc, c_sc = np.array(g_sc), np.array(p_sc), np.array(c_sc)
 
     def side(k):
-        g, p = g_sc[:, k].tolist(), p_sc[:, k].tolist()
+        g, p, c = g_sc[:, k].tolist(), p_sc[:, k].tolist(), c_sc[:, k].tolist()
+        per_run = np.asarray(g) / np.maximum(np.asarray(c), 1e-300)
         return {"relmse_guided": g, "relmse_plain": p,
                 "relmse_ratio_equal_time": float(np.mean(g) / np.mean(p)),
                 "relmse_ratio_ci95": bootstrap_ratio_ci(g, p),
-                "relmse_median_ratio": float(np.median(g) / np.median(p))}
+                "relmse_median_ratio": float(np.median(g) / np.median(p)),
+                "common_seeds": {
+                    "relmse_plain": c,
+                    "relmse_ratio_equal_time": float(np.mean(g) / np.mean(c)),
+                    "relmse_ratio_ci95": bootstrap_paired_ratio_ci(g, c),
+                    "median_of_run_ratios": float(np.median(per_run)),
+                    "runs_guided_below_plain": int((per_run < 1.0).sum())}}
 
     out = {"workload": f"glossy caustic {res}x{res}, {sc.num_triangles} tris, depth {depth}",
            "runs": runs,
            "guided": {"spp": args.guide_spp, "seconds": float(np.mean(g_t)), "adam_steps": steps},
            "plain": {"spp": spp_p, "seconds": float(np.mean(p_t))},
+           "common_seeds": "plain run i repeated with guided run i's frame offsets "
+                           "(frames r << 20 onward): the first 64 samples per pixel share "
+                           "their random numbers with the guided run; paired bootstrap",
            "reference": f"FP64 CPU oracle, {grid[2]['spp']} spp",
            "full_view": dict(side(0), pixels=f"{gw}x{gh} grid, every {gs}th pixel of the view"),
            "floor": dict(side(1), pixels=f"{int(floor.sum())} grid pixels on the metal floor"),
@@ -327,6 +348,19 @@ def bootstrap_ratio_ci(a, b, n=10000, seed=0):
     ra = a[rng.integers(0, a.size, (n, a.size))].mean(1)
     rb = b[rng.integers(0, b.size, (n, b.size))].mean(1)
     q = np.percentile(ra / rb, [2.5, 97.5])
+    return [float(q[0]), float(q[1])]
+
+
+def bootstrap_paired_ratio_ci(a, b, n=10000, seed=0):
+    """95 % percentile bootstrap interval of mean(a) / mean(b) for paired runs
+    (run i of each side shares its random numbers): run indices are resampled
+    jointly."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    if a.size < 2 or a.size != b.size:
+        return None
+    rng = np.random.default_rng(seed)
+    idx = rng.integers(0, a.size, (n, a.size))
+    q = np.percentile(a[idx].mean(1) / b[idx].mean(1), [2.5, 97.5])
     return [float(q[0]), float(q[1])]
 
 

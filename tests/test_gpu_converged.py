@@ -94,7 +94,7 @@ def test_guided_caustic_1024_unbiased(gpu_ctx):
         guide.close()
         img = pt.resolve(film, spp).cpu().numpy().astype(np.float64).reshape(res, res, 3)
         assert np.isfinite(img).all()
-        assert steps == spp * 8
+        assert steps == spp * guide.cfg.steps_per_batch
         crops.append(img[y0:y0 + wh, x0:x0 + ww])
     crops = np.stack(crops)
     # 32 x 32 blocks of the window (channel means): mean over the runs vs the
