@@ -31,6 +31,8 @@ def lib():
                                 C.c_void_p, C.c_int]
         L.wlh_serve_check.restype = C.c_uint64
         L.wlh_serve_check.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.wlh_dyn_chunks.restype = C.c_uint32
+        L.wlh_dyn_chunks.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
         _lib = L
     return _lib
 

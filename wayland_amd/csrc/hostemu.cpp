@@ -260,4 +260,10 @@ uint64_t wlh_serve_check(const uint32_t *req4, uint32_t n, uint32_t capacity, ui
     return bad;
 }
 
+// The counter advance of the run-time chunk schedule (dyn_xcd_chunks).
+uint32_t wlh_dyn_chunks(uint32_t n, uint32_t xcd, uint32_t chunk)
+{
+    return wl::dyn_xcd_chunks(n, xcd, chunk);
+}
+
 } // extern "C"

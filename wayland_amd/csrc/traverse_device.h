@@ -359,6 +359,18 @@ WL_HD bool tri_visit(const WlTriRecord *__restrict__ tris, uint32_t tri_base, ui
     return false;
 }
 
+// Chunks of XCD xcd's eighth of n rays under the run-time chunk schedule of
+// the persistent trace kernels (traverse_persistent.h, DYN = 2): eighth x is
+// [x * ceil(n / 8), (x + 1) * ceil(n / 8)) clipped to n, and a launch
+// advances XCD x's counter by exactly this many.
+WL_HD uint32_t dyn_xcd_chunks(uint32_t n, uint32_t xcd, uint32_t chunk)
+{
+    const uint32_t n8 = (n + 7u) / 8u;
+    const uint32_t lo = xcd * n8 < n ? xcd * n8 : n;
+    const uint32_t hi = lo + n8 < n ? lo + n8 : n;
+    return (hi - lo + chunk - 1u) / chunk;
+}
+
 // Next node from the current group, else from the stack; false when none.
 template <int BLOCK>
 WL_HD bool next_node(const TravRay &r, TravState &st, uint32_t *lds_stack_lane)

@@ -196,14 +196,5 @@ __device__ __forceinline__ void trav_persistent(const WlBvh8Node *__restrict__ n
     }
 }
 
-// Chunks of each XCD's eighth of n rays under DYN = 2 (the amount a launch
-// advances that XCD's counter by).
-__host__ __device__ inline uint32_t dyn_xcd_chunks(uint32_t n, uint32_t xcd, uint32_t chunk)
-{
-    const uint32_t n8 = (n + 7u) / 8u;
-    const uint32_t lo = xcd * n8 < n ? xcd * n8 : n;
-    const uint32_t hi = lo + n8 < n ? lo + n8 : n;
-    return (hi - lo + chunk - 1u) / chunk;
-}
 
 } // namespace wl
